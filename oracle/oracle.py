@@ -1,0 +1,363 @@
+"""ctypes view of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP product:
+only tests/, the smoke check and the benchmark's CPU-baseline leg import it.
+
+Arrays follow Julia's column-major layout: numpy arrays of shape (Lx, Ly) or
+(Lx, Ly, 9) in Fortran order, so element (i, j, k) sits at i + Lx*j + Lx*Ly*k
+exactly as in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("tau", C.c_double), ("mu", C.c_double), ("delta", C.c_double), ("kbt", C.c_double),
+        ("gamma", C.c_double), ("hmin", C.c_double), ("hcrit", C.c_double),
+        ("theta", C.c_double), ("g", C.c_double), ("n", C.c_int32), ("m", C.c_int32),
+        ("tdump", C.c_int64),
+    ]
+
+
+class StateC(C.Structure):
+    _fields_ = [("Lx", C.c_int32), ("Ly", C.c_int32)] + [
+        (name, _D) for name in (
+            "fout", "ftemp", "feq", "height", "velx", "vely", "vsq", "pressure",
+            "Fx", "Fy", "slipx", "slipy", "hgpx", "hgpy", "dgrad", "kbtx", "kbty")
+    ]
+
+
+class LoopOpts(C.Structure):
+    _fields_ = [
+        ("theta_field", _D), ("thermal", C.c_int32), ("body", C.c_int32),
+        ("seed", C.c_uint64), ("t0", C.c_int64), ("body_ax", C.c_double),
+        ("body_ay", C.c_double), ("body_fac", C.c_double), ("dh_series", _D),
+        ("wet_series", _I64), ("wet_thresh", C.c_double), ("mass_series", _D),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_log.restype = C.c_double
+        L.orc_log.argtypes = [C.c_double]
+        L.orc_cospi.restype = C.c_double
+        L.orc_cospi.argtypes = [C.c_double]
+        L.orc_sincospi.argtypes = [C.c_double, _D, _D]
+        L.orc_power_broad.restype = C.c_double
+        L.orc_power_broad.argtypes = [C.c_double, C.c_int]
+        L.orc_fast_93.restype = C.c_double
+        L.orc_fast_93.argtypes = [C.c_double]
+        L.orc_fast_32.restype = C.c_double
+        L.orc_fast_32.argtypes = [C.c_double]
+        L.orc_uniform.restype = C.c_double
+        L.orc_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_normal_pair.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _D, _D]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.orc_filmpressure_raw.restype = C.c_int
+        L.orc_time_loop.restype = C.c_int
+        L.orc_time_loop.argtypes = [C.POINTER(StateC), C.POINTER(Params), C.c_int64,
+                                    C.POINTER(LoopOpts)]
+        _lib = L
+    return _lib
+
+
+def _f(a, shape=None):
+    """Fresh Fortran-ordered float64 copy (inputs are never modified)."""
+    a = np.asarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        a = np.broadcast_to(a, shape)
+    return np.array(a, dtype=np.float64, order="F", copy=True)
+
+
+def _p(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+# ---------------------------------------------------------------- math
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return tuple(int(v) for v in o)
+
+
+def log(x):
+    return lib().orc_log(float(x))
+
+
+def sincospi(x):
+    s, c = C.c_double(), C.c_double()
+    lib().orc_sincospi(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def cospi(x):
+    return lib().orc_cospi(float(x))
+
+
+def normal_pair(seed, step, node, tag):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_normal_pair(seed, step, node, tag, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def uniform(seed, step, node, tag):
+    return lib().orc_uniform(seed, step, node, tag)
+
+
+def power_broad(x, n):
+    return lib().orc_power_broad(float(x), int(n))
+
+
+def fast_93(x):
+    return lib().orc_fast_93(float(x))
+
+
+def fast_32(x):
+    return lib().orc_fast_32(float(x))
+
+
+# ---------------------------------------------------------------- ops
+def circshift(src, shift):
+    src = _f(src)
+    Lx, Ly = src.shape
+    dst = np.zeros((Lx, Ly), order="F")
+    lib().orc_circshift(_p(dst), _p(src), Lx, Ly, int(shift[0]), int(shift[1]))
+    return dst
+
+
+def equilibrium(h, ux, uy, g):
+    h = _f(h)
+    Lx, Ly = h.shape
+    ux, uy = _f(ux, h.shape), _f(uy, h.shape)
+    feq = np.zeros((Lx, Ly, 9), order="F")
+    vsq = np.zeros((Lx, Ly), order="F")
+    lib().orc_equilibrium(_p(feq), _p(h), _p(ux), _p(uy), _p(vsq), C.c_double(g), Lx, Ly)
+    return feq, vsq
+
+
+def bgk_and_stream(feq, ftemp, Fx, Fy, tau):
+    """Returns (fout, ftemp) after BGKandStream! (both hold the streamed f)."""
+    feq = _f(feq)
+    Lx, Ly, _ = feq.shape
+    ftemp = np.array(_f(ftemp, feq.shape), order="F", copy=True)
+    Fx, Fy = _f(Fx, (Lx, Ly)), _f(Fy, (Lx, Ly))
+    fout = np.zeros((Lx, Ly, 9), order="F")
+    lib().orc_bgk_and_stream(_p(fout), _p(feq), _p(ftemp), _p(Fx), _p(Fy), C.c_double(tau), Lx, Ly)
+    return fout, ftemp
+
+
+def moments(fout):
+    fout = _f(fout)
+    Lx, Ly, _ = fout.shape
+    h = np.zeros((Lx, Ly), order="F")
+    ux = np.zeros((Lx, Ly), order="F")
+    uy = np.zeros((Lx, Ly), order="F")
+    lib().orc_moments(_p(h), _p(ux), _p(uy), _p(fout), Lx, Ly)
+    return h, ux, uy
+
+
+def filmpressure(h, gamma, theta, n, m, hmin, hcrit, raw=False):
+    """State form (power_broad) or, with raw=True, the fast_93/fast_32 form.
+    theta may be a scalar or an (Lx, Ly) field.  Returns p (raises
+    ValueError where the reference raises DomainError)."""
+    h = _f(h)
+    Lx, Ly = h.shape
+    p = np.zeros((Lx, Ly), order="F")
+    dgrad = np.zeros((Lx, Ly, 8), order="F")
+    tf = None
+    th = 0.0
+    if np.ndim(theta) == 0:
+        th = float(theta)
+    else:
+        tf = _f(theta, h.shape)
+    args = (_p(p), _p(h), _p(dgrad), C.c_double(gamma), C.c_double(th), _p(tf),
+            int(n), int(m), C.c_double(hmin), C.c_double(hcrit), Lx, Ly)
+    if raw:
+        if lib().orc_filmpressure_raw(*args) != 0:
+            raise ValueError("DomainError: (n, m) not in {(9,3), (3,2)}")
+    else:
+        lib().orc_filmpressure(*args)
+    return p
+
+
+def grad(f, a=None):
+    f = _f(f)
+    Lx, Ly = f.shape
+    ox = np.zeros((Lx, Ly), order="F")
+    oy = np.zeros((Lx, Ly), order="F")
+    af = None if a is None else _f(a, f.shape)
+    lib().orc_grad(_p(ox), _p(oy), _p(f), _p(af), Lx, Ly)
+    return ox, oy
+
+
+def laplacian(f, gamma):
+    f = _f(f)
+    Lx, Ly = f.shape
+    o = np.zeros((Lx, Ly), order="F")
+    lib().orc_laplacian(_p(o), _p(f), C.c_double(gamma), Lx, Ly)
+    return o
+
+
+def hgradp(p, h):
+    p = _f(p)
+    Lx, Ly = p.shape
+    h = _f(h, p.shape)
+    ox = np.zeros((Lx, Ly), order="F")
+    oy = np.zeros((Lx, Ly), order="F")
+    dgrad = np.zeros((Lx, Ly, 8), order="F")
+    lib().orc_hgradp(_p(ox), _p(oy), _p(p), _p(h), _p(dgrad), Lx, Ly)
+    return ox, oy
+
+
+def slippage(h, ux, uy, delta, mu):
+    h = _f(h)
+    Lx, Ly = h.shape
+    ux, uy = _f(ux, h.shape), _f(uy, h.shape)
+    sx = np.zeros((Lx, Ly), order="F")
+    sy = np.zeros((Lx, Ly), order="F")
+    lib().orc_slippage(_p(sx), _p(sy), _p(h), _p(ux), _p(uy), C.c_double(delta), C.c_double(mu), Lx, Ly)
+    return sx, sy
+
+
+def thermal(h, kbt, mu, delta, seed, step):
+    h = _f(h)
+    Lx, Ly = h.shape
+    kx = np.zeros((Lx, Ly), order="F")
+    ky = np.zeros((Lx, Ly), order="F")
+    lib().orc_thermal(_p(kx), _p(ky), _p(h), C.c_double(kbt), C.c_double(mu), C.c_double(delta),
+                      C.c_uint64(seed), C.c_uint64(step), Lx, Ly)
+    return kx, ky
+
+
+def inclination(Fx, Fy, h, ax, ay, fac):
+    h = _f(h)
+    Lx, Ly = h.shape
+    Fx = np.array(_f(Fx, h.shape), order="F", copy=True)
+    Fy = np.array(_f(Fy, h.shape), order="F", copy=True)
+    lib().orc_inclination(_p(Fx), _p(Fy), _p(h), C.c_double(ax), C.c_double(ay), C.c_double(fac), Lx, Ly)
+    return Fx, Fy
+
+
+def force_sum(hgpx, hgpy, slipx, slipy, kbtx=None, kbty=None):
+    hgpx = _f(hgpx)
+    Lx, Ly = hgpx.shape
+    hgpy, slipx, slipy = _f(hgpy, hgpx.shape), _f(slipx, hgpx.shape), _f(slipy, hgpx.shape)
+    kx = None if kbtx is None else _f(kbtx, hgpx.shape)
+    ky = None if kbty is None else _f(kbty, hgpx.shape)
+    Fx = np.zeros((Lx, Ly), order="F")
+    Fy = np.zeros((Lx, Ly), order="F")
+    lib().orc_force_sum(_p(Fx), _p(Fy), _p(hgpx), _p(hgpy), _p(slipx), _p(slipy), _p(kx), _p(ky), Lx, Ly)
+    return Fx, Fy
+
+
+def init_uniform(Lx, Ly, seed, base, amp):
+    h = np.zeros((Lx, Ly), order="F")
+    lib().orc_init_uniform(_p(h), Lx, Ly, C.c_uint64(seed), C.c_double(base), C.c_double(amp))
+    return h
+
+
+def init_normal(Lx, Ly, seed, base, amp):
+    h = np.zeros((Lx, Ly), order="F")
+    lib().orc_init_normal(_p(h), Lx, Ly, C.c_uint64(seed), C.c_double(base), C.c_double(amp))
+    return h
+
+
+# ---------------------------------------------------------------- driver
+FIELDS = ("fout", "ftemp", "feq", "height", "velx", "vely", "vsq", "pressure",
+          "Fx", "Fy", "slipx", "slipy", "hgpx", "hgpy", "dgrad", "kbtx", "kbty")
+
+
+class State:
+    """State{Float64,3} + kbtx/kbty as Fortran-ordered numpy arrays, zeroed
+    except height = 1 (as Sys does, src/initialize.jl:491-508)."""
+
+    def __init__(self, Lx, Ly):
+        self.Lx, self.Ly = Lx, Ly
+        for name in FIELDS:
+            if name in ("fout", "ftemp", "feq"):
+                shape = (Lx, Ly, 9)
+            elif name == "dgrad":
+                shape = (Lx, Ly, 8)
+            else:
+                shape = (Lx, Ly)
+            setattr(self, name, np.zeros(shape, order="F"))
+        self.height[...] = 1.0
+
+    def cstruct(self):
+        s = StateC()
+        s.Lx, s.Ly = self.Lx, self.Ly
+        for name in FIELDS:
+            arr = getattr(self, name)
+            assert arr.flags.f_contiguous and arr.dtype == np.float64
+            setattr(s, name, _p(arr))
+        return s
+
+    def set(self, name, value):
+        arr = getattr(self, name)
+        arr[...] = value
+
+
+def params(tau=1.0, delta=1.0, kbt=0.0, gamma=0.01, n=9, m=3, hmin=0.1, hcrit=0.05,
+           theta=1 / 9, g=0.0, tdump=100, mu=None):
+    cs = 1 / np.sqrt(3.0)
+    if mu is None:
+        mu = cs ** 2 * (tau - 0.5)
+    return Params(tau=tau, mu=mu, delta=delta, kbt=kbt, gamma=gamma, hmin=hmin, hcrit=hcrit,
+                  theta=theta, g=g, n=n, m=m, tdump=tdump)
+
+
+def time_loop(state: State, prm: Params, nsteps: int, *, theta_field=None, thermal=False,
+              seed=0, t0=0, body=None, dh=False, wetted=None, mass=False):
+    """Runs nsteps of the reference time loop.  Returns a dict of series."""
+    o = LoopOpts()
+    keep = []
+    if theta_field is not None:
+        tf = _f(theta_field, (state.Lx, state.Ly))
+        keep.append(tf)
+        o.theta_field = _p(tf)
+    o.thermal = int(bool(thermal))
+    o.seed = seed
+    o.t0 = t0
+    out = {}
+    if body is not None:
+        o.body = 1
+        o.body_ax, o.body_ay, o.body_fac = body
+    if dh:
+        out["dh"] = np.zeros(nsteps)
+        o.dh_series = _p(out["dh"])
+    if wetted is not None:
+        out["wetted"] = np.zeros(nsteps, dtype=np.int64)
+        o.wet_series = out["wetted"].ctypes.data_as(_I64)
+        o.wet_thresh = wetted
+    if mass:
+        out["mass"] = np.zeros(max(1, nsteps // max(1, prm.tdump) + 1))
+        o.mass_series = _p(out["mass"])
+    s = state.cstruct()
+    rc = lib().orc_time_loop(C.byref(s), C.byref(prm), int(nsteps), C.byref(o))
+    assert rc == 0
+    return out
