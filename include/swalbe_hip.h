@@ -1,0 +1,209 @@
+/*
+ * swalbe_hip.h -- C ABI of libswalbe_hip, the MI355X (gfx950) implementation
+ * of Swalbe.jl's 2D thin-film lattice Boltzmann time step.
+ *
+ * Plain C types only: handles, device/host pointers and sizes.  Every entry
+ * point returns an int status (SWALBE_OK = 0) and never aborts; the text of
+ * the last error of the calling thread is swalbe_last_error().
+ *
+ * Fields use the reference's layout: Julia column-major arrays, a field
+ * (Lx, Ly) stores node (i, j) (0-based) at i + Lx*j, a distribution
+ * (Lx, Ly, 9) stores population k at plane offset k*Lx*Ly.
+ *
+ * Each entry point names the reference function it replaces
+ * (paths relative to the Swalbe.jl v1.0.6 source tree).
+ */
+#ifndef SWALBE_HIP_H
+#define SWALBE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SWALBE_OK 0
+#define SWALBE_EDOMAIN 1 /* DomainError of src/pressure.jl:101-107 */
+#define SWALBE_EINVAL 2  /* bad argument (size, field id, null pointer) */
+#define SWALBE_ENOMEM 3  /* device allocation failed */
+#define SWALBE_EHIP 4    /* HIP runtime error */
+
+/* ---- element types and state kinds (Sys(sysc, device; T, kind),
+ *      src/initialize.jl:491-572) -------------------------------------- */
+#define SWALBE_F64 0
+#define SWALBE_F32 1
+#define SWALBE_SIMPLE 0
+#define SWALBE_THERMAL 1
+
+/* ---- field ids: the arrays of State{T,3} (src/initialize.jl:149-168),
+ *      State_thermal's kbtx/kbty (:182-186) and a contact-angle field θ
+ *      (time_loop(sys, state, θ), src/simulate.jl:26-45) ---------------- */
+enum swalbe_field {
+    SWALBE_FOUT = 0,
+    SWALBE_FTEMP = 1,
+    SWALBE_FEQ = 2,
+    SWALBE_HEIGHT = 3,
+    SWALBE_VELX = 4,
+    SWALBE_VELY = 5,
+    SWALBE_VSQ = 6,
+    SWALBE_PRESSURE = 7,
+    SWALBE_FX = 8,
+    SWALBE_FY = 9,
+    SWALBE_SLIPX = 10,
+    SWALBE_SLIPY = 11,
+    SWALBE_HGRADPX = 12,
+    SWALBE_HGRADPY = 13,
+    SWALBE_KBTX = 14,
+    SWALBE_KBTY = 15,
+    SWALBE_THETA = 16,
+    SWALBE_NFIELDS = 17
+};
+
+/* Taumucs{T} (src/initialize.jl:43-60).  μ is taken as given (the reference
+ * derives it as cs^2 (τ - 1/2), :49). */
+typedef struct swalbe_params {
+    int64_t Tmax, tdump;
+    double tau, cs, mu, delta, kbt, gamma;
+    int32_t n, m;
+    double hmin, hcrit, theta, g;
+} swalbe_params;
+
+typedef struct swalbe_state swalbe_state;
+
+const char *swalbe_last_error(void);
+int swalbe_version(void);
+/* Number of visible GPUs (0 on a host without one). */
+int swalbe_device_count(int32_t *n);
+
+/* ---- state lifetime: Sys(sysc, "GPU"; T, kind) (src/initialize.jl:491-572).
+ * Allocates every field on the current device, zeroed, with height = 1. */
+int swalbe_create(int32_t Lx, int32_t Ly, int32_t dtype, int32_t kind, swalbe_state **out);
+int swalbe_destroy(swalbe_state *st);
+/* FP32 states store height and distributions as deviations from a film at
+ * rest of height href (default 1); this re-encodes the resident fields.
+ * No effect on FP64 states.  g is the gravity of the rest equilibrium. */
+int swalbe_set_reference_height(swalbe_state *st, double href, double g);
+
+/* ---- field access: reads/writes of state.<field> (host <-> device copy).
+ * bytes selects the host element type: N*8 for Float64, N*4 for Float32,
+ * with N = Lx*Ly (x9 for distributions).  Synchronous. */
+int swalbe_upload(swalbe_state *st, int32_t field, const void *host, size_t bytes);
+int swalbe_download(swalbe_state *st, int32_t field, void *host, size_t bytes);
+/* state.<field> .= value (src/simulate.jl:241, test/collide.jl:19-22). */
+int swalbe_fill(swalbe_state *st, int32_t field, double value);
+int swalbe_synchronize(swalbe_state *st);
+
+/* ---- one op per reference function, on a state ------------------------ */
+/* equilibrium!(state, sys) -- src/equilibrium.jl:63-128 */
+int swalbe_equilibrium(swalbe_state *st, double g);
+/* BGKandStream!(state, sys) -- src/collide.jl:70-105, :151-161 */
+int swalbe_bgk_and_stream(swalbe_state *st, double tau);
+/* moments!(state) -- src/moments.jl:43-52, :64-73 */
+int swalbe_moments(swalbe_state *st);
+/* filmpressure!(state, sys; θ, γ, n, m, hmin, hcrit) -- src/pressure.jl:119-193
+ * (power_broad form, any n > m).  use_theta_field = 1 takes θ from the
+ * resident SWALBE_THETA field.  raw = 1 selects the raw-array method
+ * filmpressure!(p, h, dgrad, γ, θ, n, m, hmin, hcrit) (src/pressure.jl:72-117)
+ * with fast_93 / fast_32, returning SWALBE_EDOMAIN for other (n, m). */
+int swalbe_filmpressure(swalbe_state *st, double gamma, double theta, int32_t n, int32_t m,
+                        double hmin, double hcrit, int32_t use_theta_field, int32_t raw);
+/* h∇p!(state) -- src/forcing.jl:168-187, :200-221 */
+int swalbe_hgradp(swalbe_state *st);
+/* slippage!(state, sys) -- src/forcing.jl:42-66 */
+int swalbe_slippage(swalbe_state *st, double delta, double mu);
+/* thermal!(state, sys) -- src/forcing.jl:297-320.  randn! is replaced by
+ * counter-based normals keyed (seed, step, node). */
+int swalbe_thermal(swalbe_state *st, double kbt, double mu, double delta, uint64_t seed,
+                   uint64_t step);
+/* inclination!(α, state; t, tstart, tsmooth) -- src/forcing.jl:363-377, with
+ * fac = 0.5 + 0.5 tanh((t - tstart)/tsmooth) computed by the caller. */
+int swalbe_inclination(swalbe_state *st, double ax, double ay, double fac);
+/* state.Fx .= -state.h∇px .- state.slipx [.- state.kbtx]
+ * -- src/simulate.jl:18-19, scripts/Rivulet_stability.jl:123-124 */
+int swalbe_force_sum(swalbe_state *st, int32_t with_thermal);
+
+/* ---- fused time loop: time_loop(sys, state[, θ | Δh | f, measure])
+ * -- src/simulate.jl:6-96.  One kernel per step (collide + pull-stream +
+ * moments, with pressure, h∇p, slip, thermal and body forces computed from
+ * an LDS halo tile).  Steps run are t0+1 .. t0+nsteps. */
+typedef struct swalbe_step_opts {
+    int32_t use_theta_field; /* filmpressure!(state, sys, θ = field) (src/simulate.jl:35) */
+    int32_t thermal;         /* F -= thermal!(…) noise each step */
+    int32_t body;            /* inclination! callback (src/simulate.jl:89, 480) */
+    int32_t materialize;     /* last step also writes feq, vsq, Fx, Fy, slip, h∇p, kbt */
+    uint64_t seed;
+    int64_t t0;
+    double body_ax, body_ay, body_fac;
+    double wet_thresh;
+    double *dh_series;     /* host [nsteps]: max(h) - min(h) before each step (:56), or NULL */
+    int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) (measures.jl:13-17), or NULL */
+    double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
+    int32_t tile_y;        /* rows per workgroup tile: 0 = default */
+} swalbe_step_opts;
+int swalbe_time_loop(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
+                     const swalbe_step_opts *opts);
+/* Same, but enqueues only (no host series) and reports the device time of
+ * the nsteps launches measured with HIP events on the state's stream. */
+int swalbe_time_loop_timed(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
+                           const swalbe_step_opts *opts, double *ms_out);
+
+/* ---- reductions: sum(state.height) (src/simulate.jl:10), maximum/minimum
+ * (:56) and count(h .> thresh) (src/measures.jl:13-17), in FP64.
+ * out = {sum, max, min, count}.  Deterministic order. */
+int swalbe_reduce(swalbe_state *st, int32_t field, double thresh, double out[4]);
+
+/* ---- seeded initial height on the device: h = base + amp * U(0,1)
+ * (dist 0) or base + amp * N(0,1) (dist 1), Philox keyed (seed, node);
+ * the counterpart of randinterface! (src/initialvalues.jl:23-33). */
+int swalbe_init_random(swalbe_state *st, int32_t dist, uint64_t seed, double base, double amp);
+
+/* ---- raw-array twins on caller-owned device arrays (plain values of the
+ * given dtype, Julia layout), enqueued on `stream` (NULL = default stream).
+ * These replace the positional-array methods the reference exposes. */
+int swalbe_equilibrium_ptr(int32_t dtype, void *feq, const void *h, const void *ux,
+                           const void *uy, void *vsq, double g, int32_t Lx, int32_t Ly,
+                           void *stream); /* src/equilibrium.jl:63-116 */
+int swalbe_bgk_and_stream_ptr(int32_t dtype, void *fout, const void *feq, void *ftemp,
+                              const void *Fx, const void *Fy, double tau, int32_t Lx,
+                              int32_t Ly, void *stream); /* src/collide.jl:70-105 */
+int swalbe_moments_ptr(int32_t dtype, void *h, void *ux, void *uy, const void *fout, int32_t Lx,
+                       int32_t Ly, void *stream); /* src/moments.jl:43-52 */
+int swalbe_filmpressure_ptr(int32_t dtype, void *p, const void *h, double gamma, double theta,
+                            const void *theta_field, int32_t n, int32_t m, double hmin,
+                            double hcrit, int32_t raw, int32_t Lx, int32_t Ly,
+                            void *stream); /* src/pressure.jl:72-155 */
+int swalbe_grad_ptr(int32_t dtype, void *outx, void *outy, const void *f, const void *a,
+                    int32_t Lx, int32_t Ly, void *stream); /* ∇f! src/differences.jl:153-206 */
+int swalbe_laplacian_ptr(int32_t dtype, void *out, const void *f, double gamma, int32_t Lx,
+                         int32_t Ly, void *stream); /* ∇²f! src/differences.jl:57-75 */
+int swalbe_hgradp_ptr(int32_t dtype, void *hgpx, void *hgpy, const void *p, const void *h,
+                      int32_t Lx, int32_t Ly, void *stream); /* src/forcing.jl:168-187 */
+int swalbe_slippage_ptr(int32_t dtype, void *slipx, void *slipy, const void *h, const void *ux,
+                        const void *uy, double delta, double mu, int32_t Lx, int32_t Ly,
+                        void *stream); /* src/forcing.jl:42-46 */
+int swalbe_thermal_ptr(int32_t dtype, void *kbtx, void *kbty, const void *h, double kbt,
+                       double mu, double delta, uint64_t seed, uint64_t step, int32_t Lx,
+                       int32_t Ly, void *stream); /* src/forcing.jl:297-311 */
+int swalbe_inclination_ptr(int32_t dtype, void *Fx, void *Fy, const void *h, double ax,
+                           double ay, double fac, int32_t Lx, int32_t Ly,
+                           void *stream); /* src/forcing.jl:363-368 */
+/* ---- host-side draws of the same counter-based streams (initial
+ * conditions built on the host): node i of n gets the pair / uniform of
+ * counter (seed, step, i, tag).  cospi as used for κ(θ). */
+int swalbe_host_normals(uint64_t seed, uint64_t step, uint32_t tag, int64_t n, double *z0,
+                        double *z1);
+int swalbe_host_uniforms(uint64_t seed, uint64_t step, uint32_t tag, int64_t n, double *u);
+double swalbe_host_cospi(double x);
+
+/* device buffers for the raw-array forms */
+int swalbe_malloc(void **dptr, size_t bytes);
+int swalbe_free(void *dptr);
+int swalbe_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int swalbe_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWALBE_HIP_H */

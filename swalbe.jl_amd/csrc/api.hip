@@ -1,0 +1,962 @@
+// api.hip -- the C ABI of libswalbe_hip (declared in include/swalbe_hip.h).
+//
+// A swalbe_state owns every field of one film on one device and one HIP
+// stream (default priority).  Calls on one state are not re-entrant;
+// distinct states may be driven from distinct threads.
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "swalbe_internal.h"
+
+struct swalbe_state {
+  swalbe::Film film;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* partials = nullptr;  // reduction scratch, 4 doubles per block
+  int nparts = 0;
+  double* red_out = nullptr;   // 4 doubles
+};
+
+namespace swalbe {
+
+namespace {
+thread_local std::string g_last_error;
+
+constexpr int kMaxParts = 1024;
+
+inline bool is_dist(int field) {
+  return field == SWALBE_FOUT || field == SWALBE_FTEMP || field == SWALBE_FEQ;
+}
+inline bool is_height(int field) { return field == SWALBE_HEIGHT; }
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return SWALBE_EHIP;
+}
+
+Enc zero_enc() {
+  Enc e;
+  e.href = 0.f;
+  for (int k = 0; k < kQ; ++k) e.R[k] = 0.f;
+  return e;
+}
+
+// R = feq(href, u = 0) (src/equilibrium.jl:74-114 at u = 0).
+Enc make_enc(double href, double g) {
+  Enc e;
+  e.href = (float)href;
+  const double r0 = href * (1.0 - (5.0 / 6.0) * g * href);
+  const double r1 = (1.0 / 9.0) * href * (1.5 * g * href);
+  const double r5 = (1.0 / 36.0) * href * (1.5 * g * href);
+  e.R[0] = (float)r0;
+  for (int k = 1; k <= 4; ++k) e.R[k] = (float)r1;
+  for (int k = 5; k <= 8; ++k) e.R[k] = (float)r5;
+  return e;
+}
+
+}  // namespace swalbe
+
+using namespace swalbe;
+
+#define SW_CHECK_STATE(st)                   \
+  do {                                       \
+    if (!(st)) {                             \
+      set_error("null swalbe_state handle"); \
+      return SWALBE_EINVAL;                  \
+    }                                        \
+  } while (0)
+
+#define SW_HIP(call)                                  \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+namespace {
+
+// Device pointer and element count of a field; distributions have 9 planes.
+void* field_ptr(swalbe_state* st, int field, int64_t* count) {
+  Film& F = st->film;
+  *count = F.N;
+  switch (field) {
+    case SWALBE_FOUT: *count = 9 * F.N; return F.f[F.fout_buf];
+    case SWALBE_FTEMP: *count = 9 * F.N; return F.f[F.ftemp_buf];
+    case SWALBE_FEQ: *count = 9 * F.N; return F.feq;
+    case SWALBE_HEIGHT: return F.height();
+    case SWALBE_VELX: return F.velx();
+    case SWALBE_VELY: return F.vely();
+    case SWALBE_VSQ: return F.vsq;
+    case SWALBE_PRESSURE: return F.pressure;
+    case SWALBE_FX: return F.Fx;
+    case SWALBE_FY: return F.Fy;
+    case SWALBE_SLIPX: return F.slipx;
+    case SWALBE_SLIPY: return F.slipy;
+    case SWALBE_HGRADPX: return F.hgpx;
+    case SWALBE_HGRADPY: return F.hgpy;
+    case SWALBE_KBTX: return F.kbtx;
+    case SWALBE_KBTY: return F.kbty;
+    case SWALBE_THETA: return F.theta;
+    default: return nullptr;
+  }
+}
+
+// Before a write of fout or ftemp: if both names refer to one buffer, give
+// the written one the other buffer so the other name keeps its content.
+void unalias_for_write(Film& F, int field) {
+  if (F.fout_buf != F.ftemp_buf) return;
+  if (field == SWALBE_FOUT) F.fout_buf = 1 - F.ftemp_buf;
+  if (field == SWALBE_FTEMP) F.ftemp_buf = 1 - F.fout_buf;
+}
+
+// encoding of a plain value of `field` (plane k for distributions)
+inline double enc_offset(const Film& F, int field, int k) {
+  if (F.dtype != SWALBE_F32) return 0.0;
+  if (is_height(field)) return F.enc.href;
+  if (is_dist(field)) return F.enc.R[k];
+  return 0.0;
+}
+
+int alloc_field(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return SWALBE_ENOMEM;
+  }
+  return SWALBE_OK;
+}
+
+template <typename T>
+int fill_field(swalbe_state* st, int field, double value) {
+  Film& F = st->film;
+  int64_t count;
+  T* p = (T*)field_ptr(st, field, &count);
+  if (!p) {
+    set_error("field not allocated");
+    return SWALBE_EINVAL;
+  }
+  if (is_dist(field)) {
+    for (int k = 0; k < kQ; ++k)
+      launch_fill<T>(p + k * F.N, value - enc_offset(F, field, k), F.N, st->stream);
+  } else {
+    launch_fill<T>(p, value - enc_offset(F, field, 0), F.N, st->stream);
+  }
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int set_device(swalbe_state* st) {
+  hipError_t e = hipSetDevice(st->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return SWALBE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* swalbe_last_error(void) { return g_last_error.c_str(); }
+
+int swalbe_version(void) { return 10000; }
+
+int swalbe_device_count(int32_t* n) {
+  if (!n) return SWALBE_EINVAL;
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *n = (e == hipSuccess) ? c : 0;
+  return SWALBE_OK;
+}
+
+int swalbe_create(int32_t Lx, int32_t Ly, int32_t dtype, int32_t kind, swalbe_state** out) {
+  if (!out || Lx < 1 || Ly < 1 || (dtype != SWALBE_F64 && dtype != SWALBE_F32) ||
+      (kind != SWALBE_SIMPLE && kind != SWALBE_THERMAL)) {
+    set_error("swalbe_create: invalid argument");
+    return SWALBE_EINVAL;
+  }
+  if ((int64_t)Lx * Ly > (int64_t)0xFFFFFFFFLL) {
+    set_error("swalbe_create: more than 2^32 nodes (noise counters are 32-bit per node)");
+    return SWALBE_EINVAL;
+  }
+  swalbe_state* st = new swalbe_state();
+  Film& F = st->film;
+  F.Lx = Lx;
+  F.Ly = Ly;
+  F.dtype = dtype;
+  F.esz = dtype == SWALBE_F64 ? 8 : 4;
+  F.N = (int64_t)Lx * Ly;
+  F.href = dtype == SWALBE_F32 ? 1.0 : 0.0;
+  F.g_enc = 0.0;
+  F.enc = dtype == SWALBE_F32 ? make_enc(F.href, 0.0) : zero_enc();
+  hipGetDevice(&st->device);
+  int rc = SWALBE_OK;
+  const size_t fb = F.esz * F.N;
+  void** fields[] = {&F.f[0], &F.f[1], &F.feq};
+  for (void** p : fields)
+    if (rc == SWALBE_OK) rc = alloc_field(p, 9 * fb);
+  void** scal[] = {&F.hbuf[0], &F.hbuf[1], &F.uxbuf[0], &F.uxbuf[1], &F.uybuf[0], &F.uybuf[1],
+                   &F.vsq,     &F.pressure, &F.Fx,     &F.Fy,      &F.slipx,    &F.slipy,
+                   &F.hgpx,    &F.hgpy,     &F.kbtx,   &F.kbty};
+  for (void** p : scal)
+    if (rc == SWALBE_OK) rc = alloc_field(p, fb);
+  st->nparts = (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, F.N / 4096));
+  if (rc == SWALBE_OK) rc = alloc_field((void**)&st->partials, sizeof(double) * 4 * kMaxParts);
+  if (rc == SWALBE_OK) rc = alloc_field((void**)&st->red_out, sizeof(double) * 4);
+  if (rc == SWALBE_OK) {
+    hipError_t e = hipStreamCreate(&st->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamCreate");
+  }
+  if (rc != SWALBE_OK) {
+    swalbe_destroy(st);
+    return rc;
+  }
+  // Sys: every array zero, height = 1 (src/initialize.jl:492-508)
+  const int ids[] = {SWALBE_FOUT,  SWALBE_FTEMP,    SWALBE_FEQ, SWALBE_VELX, SWALBE_VELY,
+                     SWALBE_VSQ,   SWALBE_PRESSURE, SWALBE_FX,  SWALBE_FY,   SWALBE_SLIPX,
+                     SWALBE_SLIPY, SWALBE_HGRADPX,  SWALBE_HGRADPY, SWALBE_KBTX, SWALBE_KBTY};
+  F.fout_buf = 0;
+  F.ftemp_buf = 1;
+  for (int id : ids) {
+    rc = dtype == SWALBE_F64 ? fill_field<double>(st, id, 0.0) : fill_field<float>(st, id, 0.0);
+    if (rc != SWALBE_OK) break;
+  }
+  if (rc == SWALBE_OK)
+    rc = dtype == SWALBE_F64 ? fill_field<double>(st, SWALBE_HEIGHT, 1.0)
+                             : fill_field<float>(st, SWALBE_HEIGHT, 1.0);
+  if (rc == SWALBE_OK) {
+    hipError_t e = hipStreamSynchronize(st->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+  }
+  if (rc != SWALBE_OK) {
+    swalbe_destroy(st);
+    return rc;
+  }
+  (void)kind;  // every state carries kbtx/kbty; kind only documents intent
+  *out = st;
+  return SWALBE_OK;
+}
+
+int swalbe_destroy(swalbe_state* st) {
+  if (!st) return SWALBE_OK;
+  hipSetDevice(st->device);
+  if (st->stream) hipStreamSynchronize(st->stream);
+  Film& F = st->film;
+  void* ptrs[] = {F.f[0],  F.f[1],  F.feq,   F.hbuf[0], F.hbuf[1], F.uxbuf[0], F.uxbuf[1],
+                  F.uybuf[0], F.uybuf[1], F.vsq, F.pressure, F.Fx, F.Fy, F.slipx, F.slipy,
+                  F.hgpx,  F.hgpy,  F.kbtx,  F.kbty,    F.theta,   st->partials, st->red_out};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (st->stream) hipStreamDestroy(st->stream);
+  delete st;
+  return SWALBE_OK;
+}
+
+int swalbe_upload(swalbe_state* st, int32_t field, const void* host, size_t bytes) {
+  SW_CHECK_STATE(st);
+  if (field < 0 || field >= SWALBE_NFIELDS || !host) {
+    set_error("swalbe_upload: bad field id or null host pointer");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  const int64_t count = is_dist(field) ? 9 * F.N : F.N;
+  size_t hes;
+  if (bytes == (size_t)count * 8) hes = 8;
+  else if (bytes == (size_t)count * 4) hes = 4;
+  else {
+    set_error("swalbe_upload: byte count does not match the field size");
+    return SWALBE_EINVAL;
+  }
+  if (field == SWALBE_THETA && !F.theta)
+    if (int rc = alloc_field(&F.theta, F.esz * F.N)) return rc;
+  if (field == SWALBE_FOUT || field == SWALBE_FTEMP) unalias_for_write(F, field);
+  int64_t c2;
+  void* dst = field_ptr(st, field, &c2);
+  std::vector<unsigned char> tmp((size_t)count * F.esz);
+  for (int64_t i = 0; i < count; ++i) {
+    const double v = hes == 8 ? ((const double*)host)[i] : (double)((const float*)host)[i];
+    const double off = enc_offset(F, field, (int)(i / F.N));
+    if (F.esz == 8) ((double*)tmp.data())[i] = v - off;
+    else ((float*)tmp.data())[i] = (float)(v - off);
+  }
+  SW_HIP(hipStreamSynchronize(st->stream));
+  SW_HIP(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  return SWALBE_OK;
+}
+
+int swalbe_download(swalbe_state* st, int32_t field, void* host, size_t bytes) {
+  SW_CHECK_STATE(st);
+  if (field < 0 || field >= SWALBE_NFIELDS || !host) {
+    set_error("swalbe_download: bad field id or null host pointer");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  int64_t count;
+  const void* src = field_ptr(st, field, &count);
+  if (!src) {
+    set_error("swalbe_download: field not allocated (upload SWALBE_THETA first)");
+    return SWALBE_EINVAL;
+  }
+  size_t hes;
+  if (bytes == (size_t)count * 8) hes = 8;
+  else if (bytes == (size_t)count * 4) hes = 4;
+  else {
+    set_error("swalbe_download: byte count does not match the field size");
+    return SWALBE_EINVAL;
+  }
+  std::vector<unsigned char> tmp((size_t)count * F.esz);
+  SW_HIP(hipStreamSynchronize(st->stream));
+  SW_HIP(hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < count; ++i) {
+    const double raw = F.esz == 8 ? ((double*)tmp.data())[i] : (double)((float*)tmp.data())[i];
+    const double off = enc_offset(F, field, (int)(i / F.N));
+    const double v = off == 0.0 ? raw : raw + off;
+    if (hes == 8) ((double*)host)[i] = v;
+    else ((float*)host)[i] = (float)v;
+  }
+  return SWALBE_OK;
+}
+
+int swalbe_fill(swalbe_state* st, int32_t field, double value) {
+  SW_CHECK_STATE(st);
+  if (field < 0 || field >= SWALBE_NFIELDS) {
+    set_error("swalbe_fill: bad field id");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  if (field == SWALBE_THETA && !F.theta)
+    if (int rc = alloc_field(&F.theta, F.esz * F.N)) return rc;
+  if (field == SWALBE_FOUT || field == SWALBE_FTEMP) unalias_for_write(F, field);
+  return F.dtype == SWALBE_F64 ? fill_field<double>(st, field, value)
+                               : fill_field<float>(st, field, value);
+}
+
+int swalbe_synchronize(swalbe_state* st) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  SW_HIP(hipStreamSynchronize(st->stream));
+  return SWALBE_OK;
+}
+
+int swalbe_set_reference_height(swalbe_state* st, double href, double g) {
+  SW_CHECK_STATE(st);
+  Film& F = st->film;
+  if (F.dtype != SWALBE_F32) return SWALBE_OK;
+  if (!std::isfinite(href) || !std::isfinite(g)) {
+    set_error("swalbe_set_reference_height: non-finite value");
+    return SWALBE_EINVAL;
+  }
+  // re-encode resident height and distributions through the host (rare call)
+  const int ids[] = {SWALBE_HEIGHT, SWALBE_FOUT, SWALBE_FTEMP, SWALBE_FEQ};
+  std::vector<std::vector<double>> keep;
+  const bool aliased = F.fout_buf == F.ftemp_buf;
+  for (int id : ids) {
+    const int64_t count = is_dist(id) ? 9 * F.N : F.N;
+    keep.emplace_back(count);
+    if (id == SWALBE_FTEMP && aliased) continue;
+    if (int rc = swalbe_download(st, id, keep.back().data(), count * 8)) return rc;
+  }
+  F.href = href;
+  F.g_enc = g;
+  F.enc = make_enc(href, g);
+  for (size_t i = 0; i < 4; ++i) {
+    const int id = ids[i];
+    if (id == SWALBE_FTEMP && aliased) continue;
+    // write without un-aliasing: re-encoding keeps buffer identities
+    int64_t count;
+    void* dst = field_ptr(st, id, &count);
+    std::vector<unsigned char> tmp((size_t)count * 4);
+    for (int64_t j = 0; j < count; ++j)
+      ((float*)tmp.data())[j] = (float)(keep[i][j] - enc_offset(F, id, (int)(j / F.N)));
+    SW_HIP(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  }
+  return SWALBE_OK;
+}
+
+// ---------------------------------------------------------------- per-op entry points
+#define SW_DISPATCH(st, CALL_F64, CALL_F32)                 \
+  do {                                                      \
+    if ((st)->film.dtype == SWALBE_F64) {                   \
+      using T = double;                                     \
+      CALL_F64;                                             \
+    } else {                                                \
+      using T = float;                                      \
+      CALL_F32;                                             \
+    }                                                       \
+  } while (0)
+
+#define SW_BOTH(st, CALL) SW_DISPATCH(st, CALL, CALL)
+
+int swalbe_equilibrium(swalbe_state* st, double g) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  SW_BOTH(st, launch_equilibrium<T>((T*)F.feq, (const T*)F.height(), (const T*)F.velx(),
+                                    (const T*)F.vely(), (T*)F.vsq, g, F.Lx, F.Ly, F.enc,
+                                    st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_bgk_and_stream(swalbe_state* st, double tau) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  const int dst = F.fout_buf != F.ftemp_buf ? F.fout_buf : 1 - F.ftemp_buf;
+  SW_BOTH(st, launch_bgk_and_stream<T>((T*)F.f[dst], (const T*)F.feq, (const T*)F.f[F.ftemp_buf],
+                                       (const T*)F.Fx, (const T*)F.Fy, tau, F.Lx, F.Ly, F.enc,
+                                       st->stream));
+  SW_HIP(hipGetLastError());
+  F.fout_buf = F.ftemp_buf = dst;
+  return SWALBE_OK;
+}
+
+int swalbe_moments(swalbe_state* st) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  SW_BOTH(st, launch_moments<T>((T*)F.height(), (T*)F.velx(), (T*)F.vely(),
+                                (const T*)F.f[F.fout_buf], F.Lx, F.Ly, F.enc, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+static int powmode_for(int32_t n, int32_t m, int32_t raw) {
+  if (!raw) return 0;
+  if (n == 9 && m == 3) return 1;
+  if (n == 3 && m == 2) return 2;
+  return -1;
+}
+
+int swalbe_filmpressure(swalbe_state* st, double gamma, double theta, int32_t n, int32_t m,
+                        double hmin, double hcrit, int32_t use_theta_field, int32_t raw) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  const int mode = powmode_for(n, m, raw);
+  if (mode < 0) {
+    set_error("DomainError: (n, m) must be (9, 3) or (3, 2) for the raw-array filmpressure!");
+    return SWALBE_EDOMAIN;
+  }
+  if (n < 0 || m < 0) {
+    set_error("filmpressure: negative exponent");
+    return SWALBE_EINVAL;
+  }
+  if (use_theta_field && !F.theta) {
+    set_error("filmpressure: no θ field uploaded (SWALBE_THETA)");
+    return SWALBE_EINVAL;
+  }
+  const double kap = kappa(theta, n, m, hmin);
+  SW_BOTH(st, launch_filmpressure<T>((T*)F.pressure, (const T*)F.height(), gamma, kap,
+                                     use_theta_field ? (const T*)F.theta : nullptr, n, m, hmin,
+                                     hcrit, mode, F.Lx, F.Ly, F.enc, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_hgradp(swalbe_state* st) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  SW_BOTH(st, launch_grad<T>((T*)F.hgpx, (T*)F.hgpy, (const T*)F.pressure, nullptr,
+                             (const T*)F.height(), F.Lx, F.Ly, F.enc, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_slippage(swalbe_state* st, double delta, double mu) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  SW_BOTH(st, launch_slippage<T>((T*)F.slipx, (T*)F.slipy, (const T*)F.height(),
+                                 (const T*)F.velx(), (const T*)F.vely(), delta, mu, F.Lx, F.Ly,
+                                 F.enc, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_thermal(swalbe_state* st, double kbt, double mu, double delta, uint64_t seed,
+                   uint64_t step) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  SW_BOTH(st, launch_thermal<T>((T*)F.kbtx, (T*)F.kbty, (const T*)F.height(), kbt, mu, delta,
+                                seed, step, F.Lx, F.Ly, F.enc, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_inclination(swalbe_state* st, double ax, double ay, double fac) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  SW_BOTH(st, launch_inclination<T>((T*)F.Fx, (T*)F.Fy, (const T*)F.height(), ax, ay, fac, F.Lx,
+                                    F.Ly, F.enc, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_force_sum(swalbe_state* st, int32_t with_thermal) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  SW_BOTH(st, launch_force_sum<T>((T*)F.Fx, (T*)F.Fy, (const T*)F.hgpx, (const T*)F.hgpy,
+                                  (const T*)F.slipx, (const T*)F.slipy,
+                                  with_thermal ? (const T*)F.kbtx : nullptr,
+                                  with_thermal ? (const T*)F.kbty : nullptr, F.N, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+// ---------------------------------------------------------------- reductions
+static int reduce_enqueue(swalbe_state* st, const void* p, double href, double thresh,
+                          double* out_dev) {
+  Film& F = st->film;
+  SW_BOTH(st, launch_reduce<T>((const T*)p, F.N, href, thresh, st->partials, st->nparts,
+                               st->stream));
+  launch_reduce_finish(st->partials, st->nparts, out_dev, st->stream);
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_reduce(swalbe_state* st, int32_t field, double thresh, double out[4]) {
+  SW_CHECK_STATE(st);
+  if (!out || field < 0 || field >= SWALBE_NFIELDS || is_dist(field)) {
+    set_error("swalbe_reduce: bad field (scalar fields only)");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  int64_t count;
+  const void* p = field_ptr(st, field, &count);
+  if (!p) {
+    set_error("swalbe_reduce: field not allocated");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = reduce_enqueue(st, p, enc_offset(st->film, field, 0), thresh, st->red_out)) return rc;
+  SW_HIP(hipMemcpyAsync(out, st->red_out, 4 * sizeof(double), hipMemcpyDeviceToHost, st->stream));
+  SW_HIP(hipStreamSynchronize(st->stream));
+  return SWALBE_OK;
+}
+
+int swalbe_init_random(swalbe_state* st, int32_t dist, uint64_t seed, double base, double amp) {
+  SW_CHECK_STATE(st);
+  if (dist != 0 && dist != 1) {
+    set_error("swalbe_init_random: dist must be 0 (uniform) or 1 (normal)");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  const double off = F.dtype == SWALBE_F32 ? F.href : 0.0;
+  SW_BOTH(st, launch_init_random<T>((T*)F.height(), dist, seed, base, amp, off, F.Lx, F.Ly,
+                                    st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- fused time loop
+namespace {
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_init_keys(unsigned long long* p, int64_t nsteps) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nsteps;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    p[2 * i] = 0ull;
+    p[2 * i + 1] = ~0ull;
+  }
+}
+
+struct LoopBuffers {
+  unsigned long long* keys = nullptr;  // 2 per step
+  unsigned long long* wet = nullptr;   // 1 per step
+  double* mass = nullptr;              // 4 per tdump
+  ~LoopBuffers() {
+    if (keys) hipFree(keys);
+    if (wet) hipFree(wet);
+    if (mass) hipFree(mass);
+  }
+};
+
+StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, int powmode) {
+  StepParams sp{};
+  sp.gamma = p->gamma;
+  sp.kappa = kappa(p->theta, p->n, p->m, p->hmin);
+  sp.hmin = p->hmin;
+  sp.hcrit = p->hcrit;
+  sp.n = p->n;
+  sp.m = p->m;
+  sp.powmode = powmode;
+  sp.omeg = 1.0 - 1.0 / p->tau;
+  sp.itau = 1.0 / p->tau;
+  sp.g = p->g;
+  sp.c6mu = 6.0 * p->mu;
+  sp.c6d = 6.0 * p->delta;
+  sp.c3dd = 3.0 * (p->delta * p->delta);
+  sp.thermal = o && o->thermal;
+  sp.kb26 = ((2.0 * p->kbt) * p->mu) * 6.0;
+  sp.delta = p->delta;
+  sp.d3dd = (3.0 * p->delta) * p->delta;
+  sp.seed = o ? o->seed : 0;
+  sp.body = o && o->body;
+  sp.bax = o ? o->body_ax : 0.0;
+  sp.bay = o ? o->body_ay : 0.0;
+  sp.bfac = o ? o->body_fac : 0.0;
+  sp.use_theta = o && o->use_theta_field;
+  sp.read_ft = sp.omeg != 0.0;
+  sp.materialize = o && o->materialize;
+  sp.wet_thresh = o ? o->wet_thresh : 0.0;
+  return sp;
+}
+
+template <typename T>
+void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64_t t, int tile_y) {
+  Film& F = st->film;
+  const int cur = F.hu_cur;
+  const int fdst = F.fout_buf != F.ftemp_buf ? F.fout_buf : 1 - F.ftemp_buf;
+  StepIO<T> io{};
+  io.h_in = (const T*)F.hbuf[cur];
+  io.ux_in = (const T*)F.uxbuf[cur];
+  io.uy_in = (const T*)F.uybuf[cur];
+  io.ft = (const T*)F.f[F.ftemp_buf];
+  io.f_out = (T*)F.f[fdst];
+  io.h_out = (T*)F.hbuf[1 - cur];
+  io.ux_out = (T*)F.uxbuf[1 - cur];
+  io.uy_out = (T*)F.uybuf[1 - cur];
+  io.p_out = (T*)F.pressure;
+  io.theta = (const T*)F.theta;
+  launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.enc, tile_y, st->stream);
+  F.fout_buf = F.ftemp_buf = fdst;
+  F.hu_cur = 1 - cur;
+}
+
+// Recompute the intermediates of the last step from the previous moments,
+// which the ping-pong buffers still hold (same kernels as the per-op path).
+template <typename T>
+void materialize_last(swalbe_state* st, const swalbe_params* p, const swalbe_step_opts* o,
+                      uint64_t tlast) {
+  Film& F = st->film;
+  const int prev = 1 - F.hu_cur;
+  const T* h = (const T*)F.hbuf[prev];
+  const T* ux = (const T*)F.uxbuf[prev];
+  const T* uy = (const T*)F.uybuf[prev];
+  launch_grad<T>((T*)F.hgpx, (T*)F.hgpy, (const T*)F.pressure, nullptr, h, F.Lx, F.Ly, F.enc,
+                 st->stream);
+  launch_slippage<T>((T*)F.slipx, (T*)F.slipy, h, ux, uy, p->delta, p->mu, F.Lx, F.Ly, F.enc,
+                     st->stream);
+  if (o->thermal)
+    launch_thermal<T>((T*)F.kbtx, (T*)F.kbty, h, p->kbt, p->mu, p->delta, o->seed, tlast, F.Lx,
+                      F.Ly, F.enc, st->stream);
+  launch_force_sum<T>((T*)F.Fx, (T*)F.Fy, (const T*)F.hgpx, (const T*)F.hgpy, (const T*)F.slipx,
+                      (const T*)F.slipy, o->thermal ? (const T*)F.kbtx : nullptr,
+                      o->thermal ? (const T*)F.kbty : nullptr, F.N, st->stream);
+  if (o->body)
+    launch_inclination<T>((T*)F.Fx, (T*)F.Fy, h, o->body_ax, o->body_ay, o->body_fac, F.Lx, F.Ly,
+                          F.enc, st->stream);
+  launch_equilibrium<T>((T*)F.feq, h, ux, uy, (T*)F.vsq, p->g, F.Lx, F.Ly, F.enc, st->stream);
+}
+
+int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swalbe_step_opts* o,
+             double* ms_out) {
+  SW_CHECK_STATE(st);
+  if (!p || nsteps < 0) {
+    set_error("swalbe_time_loop: null params or negative step count");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  swalbe_step_opts zero{};
+  if (!o) o = &zero;
+  if (o->use_theta_field && !F.theta) {
+    set_error("swalbe_time_loop: no θ field uploaded (SWALBE_THETA)");
+    return SWALBE_EINVAL;
+  }
+  if (p->n < 0 || p->m < 0) {
+    set_error("swalbe_time_loop: negative exponent");
+    return SWALBE_EINVAL;
+  }
+  const int tile_y = o->tile_y > 0 ? o->tile_y : (F.dtype == SWALBE_F32 ? 16 : 16);
+  if (tile_y != 8 && tile_y != 16 && tile_y != 32) {
+    set_error("swalbe_time_loop: tile_y must be 8, 16 or 32");
+    return SWALBE_EINVAL;
+  }
+  const StepParams sp = make_step_params(p, o, 0);
+  LoopBuffers lb;
+  const bool want_dh = o->dh_series != nullptr, want_wet = o->wet_series != nullptr;
+  const bool want_mass = o->mass_series != nullptr && p->tdump > 0;
+  if (ms_out && (want_dh || want_wet || want_mass)) {
+    set_error("swalbe_time_loop_timed: series are not recorded by the timed variant");
+    return SWALBE_EINVAL;
+  }
+  if (want_dh) SW_HIP(hipMalloc(&lb.keys, sizeof(unsigned long long) * 2 * std::max<int64_t>(1, nsteps)));
+  if (want_wet) SW_HIP(hipMalloc(&lb.wet, sizeof(unsigned long long) * std::max<int64_t>(1, nsteps)));
+  int64_t nmass = 0;
+  for (int64_t it = 1; it <= nsteps && want_mass; ++it)
+    if ((o->t0 + it) % p->tdump == 0) ++nmass;
+  if (want_mass && nmass) SW_HIP(hipMalloc(&lb.mass, sizeof(double) * 4 * nmass));
+  if (want_dh)  // even slots hold max keys (start 0), odd slots min keys (start ~0)
+    hipLaunchKernelGGL(k_init_keys, dim3(64), dim3(256), 0, st->stream, lb.keys, nsteps);
+  if (want_wet) hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, st->stream, lb.wet, 0ull, nsteps);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms_out) {
+    SW_HIP(hipEventCreate(&e0));
+    SW_HIP(hipEventCreate(&e1));
+    SW_HIP(hipEventRecord(e0, st->stream));
+  }
+  int64_t km = 0;
+  for (int64_t it = 1; it <= nsteps; ++it) {
+    const int64_t t = o->t0 + it;
+    if (want_mass && t % p->tdump == 0) {
+      if (int rc = reduce_enqueue(st, F.height(), F.dtype == SWALBE_F32 ? F.href : 0.0, 0.0,
+                                  lb.mass + 4 * km))
+        return rc;
+      ++km;
+    }
+    StepSeries ser{want_dh ? lb.keys + 2 * (it - 1) : nullptr, want_wet ? lb.wet + (it - 1) : nullptr};
+    if (F.dtype == SWALBE_F64) enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y);
+    else enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y);
+  }
+  if (ms_out) {
+    SW_HIP(hipEventRecord(e1, st->stream));
+    SW_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SW_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_out = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  SW_HIP(hipGetLastError());
+  if (o->materialize && nsteps > 0) {
+    if (F.dtype == SWALBE_F64) materialize_last<double>(st, p, o, (uint64_t)(o->t0 + nsteps));
+    else materialize_last<float>(st, p, o, (uint64_t)(o->t0 + nsteps));
+    SW_HIP(hipGetLastError());
+  }
+  if (want_dh) {
+    std::vector<unsigned long long> keys(2 * nsteps);
+    SW_HIP(hipMemcpyAsync(keys.data(), lb.keys, keys.size() * 8, hipMemcpyDeviceToHost, st->stream));
+    SW_HIP(hipStreamSynchronize(st->stream));
+    for (int64_t i = 0; i < nsteps; ++i)
+      o->dh_series[i] = dkey_inv(keys[2 * i]) - dkey_inv(keys[2 * i + 1]);
+  }
+  if (want_wet) {
+    std::vector<unsigned long long> w(nsteps);
+    SW_HIP(hipMemcpyAsync(w.data(), lb.wet, w.size() * 8, hipMemcpyDeviceToHost, st->stream));
+    SW_HIP(hipStreamSynchronize(st->stream));
+    for (int64_t i = 0; i < nsteps; ++i) o->wet_series[i] = (int64_t)w[i];
+  }
+  if (want_mass && nmass) {
+    std::vector<double> m(4 * nmass);
+    SW_HIP(hipMemcpyAsync(m.data(), lb.mass, m.size() * 8, hipMemcpyDeviceToHost, st->stream));
+    SW_HIP(hipStreamSynchronize(st->stream));
+    for (int64_t i = 0; i < nmass; ++i) o->mass_series[i] = m[4 * i];
+  }
+  return SWALBE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swalbe_time_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps,
+                     const swalbe_step_opts* opts) {
+  return run_loop(st, p, nsteps, opts, nullptr);
+}
+
+int swalbe_time_loop_timed(swalbe_state* st, const swalbe_params* p, int64_t nsteps,
+                           const swalbe_step_opts* opts, double* ms_out) {
+  if (!ms_out) {
+    set_error("swalbe_time_loop_timed: null ms_out");
+    return SWALBE_EINVAL;
+  }
+  return run_loop(st, p, nsteps, opts, ms_out);
+}
+
+// ---------------------------------------------------------------- raw-array twins
+static int check_raw(int32_t dtype, int32_t Lx, int32_t Ly, std::initializer_list<const void*> ptrs) {
+  if ((dtype != SWALBE_F64 && dtype != SWALBE_F32) || Lx < 1 || Ly < 1) {
+    set_error("raw op: bad dtype or size");
+    return SWALBE_EINVAL;
+  }
+  for (const void* p : ptrs)
+    if (!p) {
+      set_error("raw op: null device pointer");
+      return SWALBE_EINVAL;
+    }
+  return SWALBE_OK;
+}
+
+#define SW_RAW_DISPATCH(dtype, CALL) \
+  do {                               \
+    if ((dtype) == SWALBE_F64) {     \
+      using T = double;              \
+      CALL;                          \
+    } else {                         \
+      using T = float;               \
+      CALL;                          \
+    }                                \
+  } while (0)
+
+int swalbe_equilibrium_ptr(int32_t dtype, void* feq, const void* h, const void* ux, const void* uy,
+                           void* vsq, double g, int32_t Lx, int32_t Ly, void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {feq, h, ux, uy, vsq})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_equilibrium<T>((T*)feq, (const T*)h, (const T*)ux, (const T*)uy,
+                                               (T*)vsq, g, Lx, Ly, zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_bgk_and_stream_ptr(int32_t dtype, void* fout, const void* feq, void* ftemp,
+                              const void* Fx, const void* Fy, double tau, int32_t Lx, int32_t Ly,
+                              void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {fout, feq, ftemp, Fx, Fy})) return rc;
+  if (fout == ftemp) {
+    set_error("swalbe_bgk_and_stream_ptr: fout and ftemp must be distinct arrays");
+    return SWALBE_EINVAL;
+  }
+  SW_RAW_DISPATCH(dtype, launch_bgk_and_stream<T>((T*)fout, (const T*)feq, (const T*)ftemp,
+                                                  (const T*)Fx, (const T*)Fy, tau, Lx, Ly,
+                                                  zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  const size_t bytes = (size_t)9 * Lx * Ly * (dtype == SWALBE_F64 ? 8 : 4);
+  SW_HIP(hipMemcpyAsync(ftemp, fout, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return SWALBE_OK;
+}
+
+int swalbe_moments_ptr(int32_t dtype, void* h, void* ux, void* uy, const void* fout, int32_t Lx,
+                       int32_t Ly, void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {h, ux, uy, fout})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_moments<T>((T*)h, (T*)ux, (T*)uy, (const T*)fout, Lx, Ly,
+                                           zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_filmpressure_ptr(int32_t dtype, void* p, const void* h, double gamma, double theta,
+                            const void* theta_field, int32_t n, int32_t m, double hmin, double hcrit,
+                            int32_t raw, int32_t Lx, int32_t Ly, void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {p, h})) return rc;
+  const int mode = powmode_for(n, m, raw);
+  if (mode < 0) {
+    set_error("DomainError: (n, m) must be (9, 3) or (3, 2) for the raw-array filmpressure!");
+    return SWALBE_EDOMAIN;
+  }
+  const double kap = kappa(theta, n, m, hmin);
+  SW_RAW_DISPATCH(dtype, launch_filmpressure<T>((T*)p, (const T*)h, gamma, kap,
+                                                (const T*)theta_field, n, m, hmin, hcrit, mode, Lx,
+                                                Ly, zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_grad_ptr(int32_t dtype, void* outx, void* outy, const void* f, const void* a, int32_t Lx,
+                    int32_t Ly, void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {outx, outy, f})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_grad<T>((T*)outx, (T*)outy, (const T*)f, (const T*)a, nullptr, Lx,
+                                        Ly, zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_laplacian_ptr(int32_t dtype, void* out, const void* f, double gamma, int32_t Lx,
+                         int32_t Ly, void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {out, f})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_laplacian<T>((T*)out, (const T*)f, gamma, Lx, Ly,
+                                             (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_hgradp_ptr(int32_t dtype, void* hgpx, void* hgpy, const void* p, const void* h,
+                      int32_t Lx, int32_t Ly, void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {hgpx, hgpy, p, h})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_grad<T>((T*)hgpx, (T*)hgpy, (const T*)p, nullptr, (const T*)h, Lx,
+                                        Ly, zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_slippage_ptr(int32_t dtype, void* slipx, void* slipy, const void* h, const void* ux,
+                        const void* uy, double delta, double mu, int32_t Lx, int32_t Ly,
+                        void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {slipx, slipy, h, ux, uy})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_slippage<T>((T*)slipx, (T*)slipy, (const T*)h, (const T*)ux,
+                                            (const T*)uy, delta, mu, Lx, Ly, zero_enc(),
+                                            (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_thermal_ptr(int32_t dtype, void* kbtx, void* kbty, const void* h, double kbt, double mu,
+                       double delta, uint64_t seed, uint64_t step, int32_t Lx, int32_t Ly,
+                       void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {kbtx, kbty, h})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_thermal<T>((T*)kbtx, (T*)kbty, (const T*)h, kbt, mu, delta, seed,
+                                           step, Lx, Ly, zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_inclination_ptr(int32_t dtype, void* Fx, void* Fy, const void* h, double ax, double ay,
+                           double fac, int32_t Lx, int32_t Ly, void* stream) {
+  if (int rc = check_raw(dtype, Lx, Ly, {Fx, Fy, h})) return rc;
+  SW_RAW_DISPATCH(dtype, launch_inclination<T>((T*)Fx, (T*)Fy, (const T*)h, ax, ay, fac, Lx, Ly,
+                                               zero_enc(), (hipStream_t)stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_host_normals(uint64_t seed, uint64_t step, uint32_t tag, int64_t n, double* z0,
+                        double* z1) {
+  if (n < 0 || (n > 0 && (!z0 || !z1))) {
+    set_error("swalbe_host_normals: bad arguments");
+    return SWALBE_EINVAL;
+  }
+  for (int64_t i = 0; i < n; ++i) normal_pair(seed, step, (uint32_t)i, tag, &z0[i], &z1[i]);
+  return SWALBE_OK;
+}
+
+int swalbe_host_uniforms(uint64_t seed, uint64_t step, uint32_t tag, int64_t n, double* u) {
+  if (n < 0 || (n > 0 && !u)) {
+    set_error("swalbe_host_uniforms: bad arguments");
+    return SWALBE_EINVAL;
+  }
+  for (int64_t i = 0; i < n; ++i) u[i] = uniform(seed, step, (uint32_t)i, tag);
+  return SWALBE_OK;
+}
+
+double swalbe_host_cospi(double x) { return cospi(x); }
+
+int swalbe_malloc(void** dptr, size_t bytes) {
+  if (!dptr) return SWALBE_EINVAL;
+  return alloc_field(dptr, bytes);
+}
+
+int swalbe_free(void* dptr) {
+  if (dptr) SW_HIP(hipFree(dptr));
+  return SWALBE_OK;
+}
+
+int swalbe_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  SW_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return SWALBE_OK;
+}
+
+int swalbe_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  SW_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return SWALBE_OK;
+}
+
+}  // extern "C"

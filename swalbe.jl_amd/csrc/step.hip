@@ -1,0 +1,422 @@
+// step.hip -- the fused D2Q9 thin-film time step (one kernel per step).
+//
+// One reference step (src/simulate.jl:15-22) is
+//   p = filmpressure(h)          3x3 stencil of h
+//   F = -h∇p - slip(h,u) [-kbt] [+body]   3x3 stencil of p
+//   fo = (1-1/τ) ftemp + feq(h,u)/τ + S(F)
+//   f_{t+1,k}(x) = fo_k(x - c_k)  (pull stream)
+//   h, u = moments(f_{t+1})
+// so an output node needs F and u on a 1-node halo, p on 2 and h on 3.
+// A workgroup owns a 64 x TY tile: it stages h (3-halo) and u (1-halo) in
+// LDS, computes p on the 2-halo and F on the 1-halo in LDS, then every
+// thread collides the nine source populations of its output nodes, writes
+// the streamed f and the new moments.  HBM traffic per lattice update is
+// h, u in (+ f_t when τ ≠ 1) and f, h, u, p out.
+//
+// FP64: every value equals the unfused reference sequence bit for bit (same
+// expressions, same order, no contraction).  FP32: height and distributions
+// are handled as deviations from a rest film of height href (Enc).
+#include "swalbe_internal.h"
+
+namespace swalbe {
+
+namespace {
+
+constexpr int TX = 64;        // tile width = one wavefront
+constexpr int NT = 256;       // threads per workgroup (4 wavefronts)
+
+__device__ __forceinline__ int wrap1(int a, int L) {
+  return a < 0 ? a + L : (a >= L ? a - L : a);
+}
+__device__ __forceinline__ int wrapm(int a, int L) {
+  a %= L;
+  return a < 0 ? a + L : a;
+}
+
+template <typename T>
+__device__ __forceinline__ T powlaw(T a, int n, int m, int mode) {
+  if (mode == 1) {
+    const T t = (a * a) * a;
+    return ((t * t) * t) - t;
+  }
+  if (mode == 2) return ((a * a) * a) - (a * a);
+  T tn = T(1), tm = T(1);
+  for (int i = 0; i < n; ++i) tn *= a;
+  for (int i = 0; i < m; ++i) tm *= a;
+  return tn - tm;
+}
+
+// Equilibrium population k at a node (src/equilibrium.jl:71-114), plain values.
+template <typename T>
+__device__ __forceinline__ T feq_k(int k, T h, T u, T v, T usq, T c56g, T g0) {
+  const T w1 = T(1.0 / 9.0), w5 = T(1.0 / 36.0);
+  T a;
+  switch (k) {
+    case 0: return h * ((T(1) - c56g * h) - T(2.0 / 3.0) * usq);
+    case 1: return w1 * h * (((g0 * h + T(3) * u) + T(4.5) * (u * u)) - T(1.5) * usq);
+    case 2: return w1 * h * (((g0 * h + T(3) * v) + T(4.5) * (v * v)) - T(1.5) * usq);
+    case 3: return w1 * h * (((g0 * h - T(3) * u) + T(4.5) * (u * u)) - T(1.5) * usq);
+    case 4: return w1 * h * (((g0 * h - T(3) * v) + T(4.5) * (v * v)) - T(1.5) * usq);
+    case 5: a = u + v; return w5 * h * (((g0 * h + T(3) * a) + T(4.5) * (a * a)) - T(1.5) * usq);
+    case 6: a = v - u; return w5 * h * (((g0 * h + T(3) * a) + T(4.5) * (a * a)) - T(1.5) * usq);
+    case 7: a = u + v; return w5 * h * (((g0 * h - T(3) * a) + T(4.5) * (a * a)) - T(1.5) * usq);
+    default: a = u - v; return w5 * h * (((g0 * h + T(3) * a) + T(4.5) * (a * a)) - T(1.5) * usq);
+  }
+}
+
+// Deviation feq_k(h, u) - feq_k(href, 0) with h = href + dh, free of the
+// cancellation of the plain form (FP32 path).
+__device__ __forceinline__ float feq_dev(int k, float dh, float h, float href, float u, float v,
+                                         float usq, float c56g, float g0) {
+  const float gg = dh * (h + href);  // h^2 - href^2
+  float a, s;
+  switch (k) {
+    case 0: return (dh - c56g * gg) - (2.0f / 3.0f) * h * usq;
+    case 1: a = u; s = 1.f; break;
+    case 2: a = v; s = 1.f; break;
+    case 3: a = u; s = -1.f; break;
+    case 4: a = v; s = -1.f; break;
+    case 5: a = u + v; s = 1.f; break;
+    case 6: a = v - u; s = 1.f; break;
+    case 7: a = u + v; s = -1.f; break;
+    default: a = u - v; s = 1.f; break;
+  }
+  const float w = k <= 4 ? (1.0f / 9.0f) : (1.0f / 36.0f);
+  return w * (g0 * gg + h * ((s * 3.0f * a + 4.5f * (a * a)) - 1.5f * usq));
+}
+
+template <typename T>
+__device__ __forceinline__ T force_term(int k, T Fx, T Fy) {
+  const T c3 = T(1.0 / 3.0), c24 = T(1.0 / 24.0);
+  switch (k) {
+    case 1: return c3 * Fx;
+    case 2: return c3 * Fy;
+    case 3: return -(c3 * Fx);
+    case 4: return -(c3 * Fy);
+    case 5: return c24 * (Fx + Fy);
+    case 6: return c24 * (Fy - Fx);
+    case 7: return -(c24 * (Fx + Fy));
+    case 8: return c24 * (Fx - Fy);
+    default: return T(0);
+  }
+}
+
+template <typename T, int TY, bool READ_FT, bool THERMAL>
+__global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSeries ser,
+                                             uint64_t step, int Lx, int Ly, int ntx, int ntiles,
+                                             int small, Enc enc) {
+  constexpr bool DEV = sizeof(T) == 4;
+  constexpr int HC = TX + 6, HR = TY + 6;  // h tile, 3-halo
+  constexpr int PC = TX + 4, PR = TY + 4;  // p tile, 2-halo
+  constexpr int UC = TX + 2, UR = TY + 2;  // u and F tiles, 1-halo
+  __shared__ T sh_h[HR * HC];
+  __shared__ T sh_p[PR * PC];
+  __shared__ T sh_ux[UR * UC], sh_uy[UR * UC];
+  __shared__ T sh_fx[UR * UC], sh_fy[UR * UC];
+  __shared__ unsigned long long red_k[2 * (NT / 64)];
+  __shared__ unsigned int red_c[NT / 64];
+
+  // XCD-aware tile order: the 8 XCDs receive workgroups round robin, so map
+  // consecutive block ids of one XCD to neighbouring tiles (shared halos in
+  // that XCD's L2).
+  int tile = blockIdx.x;
+  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+  const int x0 = (tile % ntx) * TX;
+  const int y0 = (tile / ntx) * TY;
+  const int tid = threadIdx.x;
+  const int64_t N = (int64_t)Lx * Ly;
+
+  // ---- phase A: stage h (3-halo) and u (1-halo)
+  for (int idx = tid; idx < HR * HC; idx += NT) {
+    const int r = idx / HC, c = idx - r * HC;
+    int gx = x0 - 3 + c, gy = y0 - 3 + r;
+    gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
+    gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
+    sh_h[idx] = io.h_in[gx + (int64_t)Lx * gy];
+  }
+  for (int idx = tid; idx < UR * UC; idx += NT) {
+    const int r = idx / UC, c = idx - r * UC;
+    int gx = x0 - 1 + c, gy = y0 - 1 + r;
+    gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
+    gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
+    const int64_t g = gx + (int64_t)Lx * gy;
+    sh_ux[idx] = io.ux_in[g];
+    sh_uy[idx] = io.uy_in[g];
+  }
+  __syncthreads();
+
+  const T gamma = (T)sp.gamma, hmin = (T)sp.hmin, hcrit = (T)sp.hcrit;
+  const T href = DEV ? (T)enc.href : T(0);
+
+  // ---- phase B: film pressure on the 2-halo (src/pressure.jl:119-155)
+  for (int idx = tid; idx < PR * PC; idx += NT) {
+    const int r = idx / PC, c = idx - r * PC;
+    const T* hc = &sh_h[(r + 1) * HC + (c + 1)];
+    const T hip = hc[-1], him = hc[1], hjp = hc[-HC], hjm = hc[HC];
+    const T hipjp = hc[-HC - 1], himjp = hc[-HC + 1], himjm = hc[HC + 1], hipjm = hc[HC - 1];
+    const T L = (T(2.0 / 3.0) * (((hjp + hip) + him) + hjm) +
+                 T(1.0 / 6.0) * (((hipjp + himjp) + himjm) + hipjm)) -
+                T(10.0 / 3.0) * hc[0];
+    const T hh = DEV ? href + hc[0] : hc[0];
+    T kap = (T)sp.kappa;
+    if (sp.use_theta) {
+      int gx = x0 - 2 + c, gy = y0 - 2 + r;
+      gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
+      gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
+      kap = (T)kappa((double)io.theta[gx + (int64_t)Lx * gy], sp.n, sp.m, sp.hmin);
+    }
+    const T a = hmin / (hh + hcrit);
+    const T p0 = -gamma * (kap * powlaw(a, sp.n, sp.m, sp.powmode));
+    sh_p[idx] = p0 - gamma * L;
+  }
+  __syncthreads();
+
+  // ---- phase C: forces on the 1-halo (src/forcing.jl:42-46, 168-187, 297-311, 363-368)
+  {
+    const T c6mu = (T)sp.c6mu, c6d = (T)sp.c6d, c3dd = (T)sp.c3dd;
+    for (int idx = tid; idx < UR * UC; idx += NT) {
+      const int r = idx / UC, c = idx - r * UC;
+      const T* pc = &sh_p[(r + 1) * PC + (c + 1)];
+      const T fip = pc[-1], fim = pc[1], fjp = pc[-PC], fjm = pc[PC];
+      const T fipjp = pc[-PC - 1], fimjp = pc[-PC + 1], fimjm = pc[PC + 1], fipjm = pc[PC - 1];
+      const T hs = sh_h[(r + 2) * HC + (c + 2)];
+      const T hh = DEV ? href + hs : hs;
+      const T gx = T(-1.0 / 3.0) * (fip - fim) - T(1.0 / 12.0) * (((fipjp - fimjp) - fimjm) + fipjm);
+      const T gy = T(-1.0 / 3.0) * (fjp - fjm) - T(1.0 / 12.0) * (((fipjp + fimjp) - fimjm) - fipjm);
+      const T hgx = hh * gx, hgy = hh * gy;
+      const T den = (T(2) * (hh * hh) + c6d * hh) + c3dd;
+      const T sx = (c6mu * hh * sh_ux[idx]) / den;
+      const T sy = (c6mu * hh * sh_uy[idx]) / den;
+      T Fx = -hgx - sx;
+      T Fy = -hgy - sy;
+      if constexpr (THERMAL) {
+        int gxx = x0 - 1 + c, gyy = y0 - 1 + r;
+        gxx = small ? wrapm(gxx, Lx) : wrap1(gxx, Lx);
+        gyy = small ? wrapm(gyy, Ly) : wrap1(gyy, Ly);
+        double z0, z1;
+        normal_pair(sp.seed, step, (uint32_t)(gxx + (int64_t)Lx * gyy), kTagThermal, &z0, &z1);
+        const T amp = sqrt((T)sp.kb26 * hh /
+                           (((T(2) * hh) * hh + (T(6) * hh) * (T)sp.delta) + (T)sp.d3dd));
+        Fx = Fx - (T)z0 * amp;
+        Fy = Fy - (T)z1 * amp;
+      }
+      if (sp.body) {
+        Fx = Fx + (hh * (T)sp.bax) * (T)sp.bfac;
+        Fy = Fy + (hh * (T)sp.bay) * (T)sp.bfac;
+      }
+      sh_fx[idx] = Fx;
+      sh_fy[idx] = Fy;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase D: collide the source populations, pull-stream, moments
+  const int tx = tid & (TX - 1);
+  const int ty = tid >> 6;
+  const T omeg = (T)sp.omeg, itau = (T)sp.itau;
+  const T c56g = (T)((5.0 / 6.0) * sp.g), g0 = (T)(1.5 * sp.g);
+  double lmax = -__builtin_inf(), lmin = __builtin_inf();
+  unsigned int lcnt = 0;
+  const int gxo = x0 + tx;
+#pragma unroll 1
+  for (int row = ty; row < TY; row += NT / TX) {
+    const int gyo = y0 + row;
+    if (gxo >= Lx || gyo >= Ly) continue;
+    const int64_t xo = gxo + (int64_t)Lx * gyo;
+    T fo[kQ];
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+      const int su = (row + 1 - cy(k)) * UC + (tx + 1 - cx(k));
+      const T hs = sh_h[(row + 3 - cy(k)) * HC + (tx + 3 - cx(k))];
+      const T us = sh_ux[su], vs = sh_uy[su];
+      const T usq = us * us + vs * vs;
+      T f;
+      if constexpr (DEV) {
+        f = itau * feq_dev(k, hs, href + hs, href, us, vs, usq, c56g, g0);
+      } else {
+        f = itau * feq_k<T>(k, hs, us, vs, usq, c56g, g0);
+      }
+      if constexpr (READ_FT) {
+        int sx = gxo - cx(k), sy = gyo - cy(k);
+        sx = small ? wrapm(sx, Lx) : wrap1(sx, Lx);
+        sy = small ? wrapm(sy, Ly) : wrap1(sy, Ly);
+        f = omeg * io.ft[k * N + sx + (int64_t)Lx * sy] + f;
+      }
+      if (k > 0) f = f + force_term<T>(k, sh_fx[su], sh_fy[su]);
+      fo[k] = f;
+      io.f_out[k * N + xo] = f;
+    }
+    T s = fo[0];
+#pragma unroll
+    for (int k = 1; k < kQ; ++k) s = s + fo[k];
+    const T hnew = DEV ? href + s : s;
+    io.h_out[xo] = s;
+    io.ux_out[xo] = (((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7]) + fo[8]) / hnew;
+    io.uy_out[xo] = (((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7]) - fo[8]) / hnew;
+    io.p_out[xo] = sh_p[(row + 2) * PC + (tx + 2)];
+    if (ser.dh_keys || ser.wet) {
+      const T hc = sh_h[(row + 3) * HC + (tx + 3)];
+      const double hv = DEV ? (double)href + (double)hc : (double)hc;
+      lmax = hv > lmax ? hv : lmax;
+      lmin = hv < lmin ? hv : lmin;
+      lcnt += hv > sp.wet_thresh ? 1u : 0u;
+    }
+  }
+
+  // ---- per-step series: max/min/count of h_t over the domain (exact)
+  if (ser.dh_keys || ser.wet) {
+    unsigned long long kmax = dkey(lmax), kmin = dkey(lmin);
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long a = __shfl_xor(kmax, o), b = __shfl_xor(kmin, o);
+      kmax = a > kmax ? a : kmax;
+      kmin = b < kmin ? b : kmin;
+      lcnt += __shfl_xor(lcnt, o);
+    }
+    if ((tid & 63) == 0) {
+      red_k[2 * ty] = kmax;
+      red_k[2 * ty + 1] = kmin;
+      red_c[ty] = lcnt;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < NT / 64; ++w) {
+        kmax = red_k[2 * w] > kmax ? red_k[2 * w] : kmax;
+        kmin = red_k[2 * w + 1] < kmin ? red_k[2 * w + 1] : kmin;
+        lcnt += red_c[w];
+      }
+      if (ser.dh_keys) {
+        atomicMax(&ser.dh_keys[0], kmax);
+        atomicMin(&ser.dh_keys[1], kmin);
+      }
+      if (ser.wet) atomicAdd(&ser.wet[0], (unsigned long long)lcnt);
+    }
+  }
+}
+
+template <typename T, int TY>
+void launch_ty(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step, int Lx,
+               int Ly, Enc enc, hipStream_t s) {
+  const int ntx = (Lx + TX - 1) / TX, nty = (Ly + TY - 1) / TY;
+  const int ntiles = ntx * nty;
+  const int small = (Lx < TX + 6 || Ly < TY + 6) ? 1 : 0;
+  dim3 grid(ntiles), block(NT);
+  if (sp.read_ft) {
+    if (sp.thermal)
+      hipLaunchKernelGGL((k_step<T, TY, true, true>), grid, block, 0, s, io, sp, ser, step, Lx, Ly,
+                         ntx, ntiles, small, enc);
+    else
+      hipLaunchKernelGGL((k_step<T, TY, true, false>), grid, block, 0, s, io, sp, ser, step, Lx,
+                         Ly, ntx, ntiles, small, enc);
+  } else {
+    if (sp.thermal)
+      hipLaunchKernelGGL((k_step<T, TY, false, true>), grid, block, 0, s, io, sp, ser, step, Lx,
+                         Ly, ntx, ntiles, small, enc);
+    else
+      hipLaunchKernelGGL((k_step<T, TY, false, false>), grid, block, 0, s, io, sp, ser, step, Lx,
+                         Ly, ntx, ntiles, small, enc);
+  }
+}
+
+// ---------------------------------------------------------------- reductions
+constexpr int kRedBlock = 256;
+
+template <typename T>
+__global__ void k_reduce(const T* __restrict__ h, int64_t N, double href, double thresh,
+                         double* __restrict__ partials) {
+  // each block reduces a contiguous chunk in a fixed order: deterministic
+  __shared__ double s_sum[kRedBlock / 64], s_max[kRedBlock / 64], s_min[kRedBlock / 64];
+  __shared__ double s_cnt[kRedBlock / 64];
+  const int64_t per = (N + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = blockIdx.x * per;
+  const int64_t b1 = b0 + per < N ? b0 + per : N;
+  double sum = 0.0, mx = -__builtin_inf(), mn = __builtin_inf(), cnt = 0.0;
+  for (int64_t x = b0 + threadIdx.x; x < b1; x += kRedBlock) {
+    const double v = href + (double)h[x];
+    sum += v;
+    mx = v > mx ? v : mx;
+    mn = v < mn ? v : mn;
+    cnt += v > thresh ? 1.0 : 0.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o);
+    const double a = __shfl_xor(mx, o), b = __shfl_xor(mn, o);
+    mx = a > mx ? a : mx;
+    mn = b < mn ? b : mn;
+    cnt += __shfl_xor(cnt, o);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_sum[w] = sum;
+    s_max[w] = mx;
+    s_min[w] = mn;
+    s_cnt[w] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kRedBlock / 64; ++i) {
+      sum += s_sum[i];
+      mx = s_max[i] > mx ? s_max[i] : mx;
+      mn = s_min[i] < mn ? s_min[i] : mn;
+      cnt += s_cnt[i];
+    }
+    partials[4 * blockIdx.x + 0] = sum;
+    partials[4 * blockIdx.x + 1] = mx;
+    partials[4 * blockIdx.x + 2] = mn;
+    partials[4 * blockIdx.x + 3] = cnt;
+  }
+}
+
+__global__ void k_reduce_finish(const double* __restrict__ partials, int nparts,
+                                double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double sum = 0.0, mx = -__builtin_inf(), mn = __builtin_inf(), cnt = 0.0;
+  for (int i = 0; i < nparts; ++i) {
+    sum += partials[4 * i];
+    mx = partials[4 * i + 1] > mx ? partials[4 * i + 1] : mx;
+    mn = partials[4 * i + 2] < mn ? partials[4 * i + 2] : mn;
+    cnt += partials[4 * i + 3];
+  }
+  out[0] = sum;
+  out[1] = mx;
+  out[2] = mn;
+  out[3] = cnt;
+}
+
+}  // namespace
+
+template <typename T>
+void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step,
+                 int Lx, int Ly, Enc enc, int tile_y, hipStream_t s) {
+  switch (tile_y) {
+    case 8: launch_ty<T, 8>(io, sp, ser, step, Lx, Ly, enc, s); break;
+    case 32:
+      if constexpr (sizeof(T) == 4) {
+        launch_ty<T, 32>(io, sp, ser, step, Lx, Ly, enc, s);
+        break;
+      }
+      [[fallthrough]];
+    default: launch_ty<T, 16>(io, sp, ser, step, Lx, Ly, enc, s); break;
+  }
+}
+
+template <typename T>
+void launch_reduce(const T* h, int64_t N, double href, double thresh, double* partials,
+                   int nparts, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce<T>, dim3(nparts), dim3(kRedBlock), 0, s, h, N, href, thresh,
+                     partials);
+}
+
+void launch_reduce_finish(const double* partials, int nparts, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(64), 0, s, partials, nparts, out);
+}
+
+template void launch_step<double>(const StepIO<double>&, const StepParams&, StepSeries, uint64_t,
+                                  int, int, Enc, int, hipStream_t);
+template void launch_step<float>(const StepIO<float>&, const StepParams&, StepSeries, uint64_t,
+                                 int, int, Enc, int, hipStream_t);
+template void launch_reduce<double>(const double*, int64_t, double, double, double*, int,
+                                    hipStream_t);
+template void launch_reduce<float>(const float*, int64_t, double, double, double*, int,
+                                   hipStream_t);
+
+}  // namespace swalbe

@@ -1,0 +1,158 @@
+// swalbe_internal.h -- device-side state layout and the launchers shared by
+// the translation units of libswalbe_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "swalbe_hip.h"
+#include "swalbe_math.h"
+
+namespace swalbe {
+
+// FP32 states store height and distributions as deviations from a rest
+// state of reference height href: dh = h - href and f'_k = f_k - R_k with
+// R = feq(href, u = 0).  FP64 states use href = 0 and R = 0 (plain values).
+struct Enc {
+  float href;
+  float R[kQ];
+};
+
+Enc make_enc(double href, double g);
+Enc zero_enc();
+
+// Layout of a device-resident film: Julia column-major fields, x fastest.
+struct Film {
+  int Lx = 0, Ly = 0;
+  int dtype = SWALBE_F64;
+  size_t esz = 8;
+  int64_t N = 0;
+  double href = 0.0, g_enc = 0.0;  // encoding parameters (FP32 only)
+  Enc enc{};
+  void* f[2] = {nullptr, nullptr};    // distribution ping-pong buffers
+  int fout_buf = 0, ftemp_buf = 1;    // which buffer is fout / ftemp
+  void* hbuf[2] = {nullptr, nullptr};
+  void* uxbuf[2] = {nullptr, nullptr};
+  void* uybuf[2] = {nullptr, nullptr};
+  int hu_cur = 0;                     // which h/ux/uy buffer holds the current moments
+  void* feq = nullptr;
+  void* vsq = nullptr;
+  void* pressure = nullptr;
+  void* Fx = nullptr;
+  void* Fy = nullptr;
+  void* slipx = nullptr;
+  void* slipy = nullptr;
+  void* hgpx = nullptr;
+  void* hgpy = nullptr;
+  void* kbtx = nullptr;
+  void* kbty = nullptr;
+  void* theta = nullptr;              // θ field, allocated on first upload
+  void* height() const { return hbuf[hu_cur]; }
+  void* velx() const { return uxbuf[hu_cur]; }
+  void* vely() const { return uybuf[hu_cur]; }
+};
+
+// ---------------------------------------------------------------- per-op launchers
+// All arrays are device pointers of element type T (double or float); `enc`
+// decodes FP32 deviation storage (zero_enc() for plain arrays).
+template <typename T>
+void launch_equilibrium(T* feq, const T* h, const T* ux, const T* uy, T* vsq, double g, int Lx,
+                        int Ly, Enc enc, hipStream_t s);
+template <typename T>
+void launch_bgk_and_stream(T* fdst, const T* feq, const T* ftemp, const T* Fx, const T* Fy,
+                           double tau, int Lx, int Ly, Enc enc, hipStream_t s);
+template <typename T>
+void launch_moments(T* h, T* ux, T* uy, const T* f, int Lx, int Ly, Enc enc, hipStream_t s);
+template <typename T>
+void launch_filmpressure(T* p, const T* h, double gamma, double kappa_scalar, const T* theta,
+                         int n, int m, double hmin, double hcrit, int powmode, int Lx, int Ly,
+                         Enc enc, hipStream_t s);
+template <typename T>
+void launch_grad(T* ox, T* oy, const T* f, const T* a, const T* h, int Lx, int Ly, Enc enc,
+                 hipStream_t s);
+template <typename T>
+void launch_laplacian(T* out, const T* f, double gamma, int Lx, int Ly, hipStream_t s);
+template <typename T>
+void launch_slippage(T* sx, T* sy, const T* h, const T* ux, const T* uy, double delta, double mu,
+                     int Lx, int Ly, Enc enc, hipStream_t s);
+template <typename T>
+void launch_thermal(T* kx, T* ky, const T* h, double kbt, double mu, double delta, uint64_t seed,
+                    uint64_t step, int Lx, int Ly, Enc enc, hipStream_t s);
+template <typename T>
+void launch_inclination(T* Fx, T* Fy, const T* h, double ax, double ay, double fac, int Lx,
+                        int Ly, Enc enc, hipStream_t s);
+template <typename T>
+void launch_force_sum(T* Fx, T* Fy, const T* hgpx, const T* hgpy, const T* sx, const T* sy,
+                      const T* kx, const T* ky, int64_t N, hipStream_t s);
+template <typename T>
+void launch_fill(T* p, double v, int64_t N, hipStream_t s);
+template <typename T>
+void launch_init_random(T* h, int dist, uint64_t seed, double base, double amp, double href,
+                        int Lx, int Ly, hipStream_t s);
+
+// ---------------------------------------------------------------- fused step
+struct StepParams {
+  double gamma, kappa, hmin, hcrit;
+  int n, m, powmode;          // powmode 0: power_broad, 1: fast_93, 2: fast_32
+  double omeg, itau, g;
+  double c6mu, c6d, c3dd;     // slip constants 6μ, 6δ, 3δ²
+  int thermal;
+  double kb26, delta, d3dd;   // ((2 kbt) μ) 6, δ, (3δ)δ
+  uint64_t seed;
+  int body;
+  double bax, bay, bfac;
+  int use_theta;
+  int read_ft;
+  int materialize;
+  double wet_thresh;
+};
+
+struct StepSeries {
+  unsigned long long* dh_keys;  // [2] ordered keys of max / min h, or null
+  unsigned long long* wet;      // [1] count of h > wet_thresh, or null
+};
+
+template <typename T>
+struct StepIO {
+  const T* h_in;
+  const T* ux_in;
+  const T* uy_in;
+  const T* ft;      // f_t (ftemp); read when read_ft
+  T* f_out;
+  T* h_out;
+  T* ux_out;
+  T* uy_out;
+  T* p_out;
+  const T* theta;
+  // optional materialised intermediates (last step of a loop)
+  T *feq, *vsq, *Fx, *Fy, *slipx, *slipy, *hgpx, *hgpy, *kbtx, *kbty;
+};
+
+template <typename T>
+void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step,
+                 int Lx, int Ly, Enc enc, int tile_y, hipStream_t s);
+
+// Deterministic FP64 reductions of a field: sum, max, min, count(> thresh).
+template <typename T>
+void launch_reduce(const T* h, int64_t N, double href, double thresh, double* partials,
+                   int nparts, hipStream_t s);
+void launch_reduce_finish(const double* partials, int nparts, double* out, hipStream_t s);
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// Ordered 64-bit key of a double (monotone in the value) and back.
+SW_HD unsigned long long dkey(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+SW_HD double dkey_inv(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __builtin_bit_cast(double, b);
+}
+
+}  // namespace swalbe

@@ -1,0 +1,25 @@
+"""swalbe -- Swalbe.jl's 2D thin-film lattice Boltzmann step on MI355X.
+
+The Python face of libswalbe_hip.so: the reference's parameter structs, a
+GPU-resident state and the ops / time loop / run_* drivers by the
+reference's names (without the trailing `!`).
+
+    import swalbe as Swalbe
+    sys = Swalbe.SysConst(Lx=256, Ly=256, param=Swalbe.Taumucs(Tmax=1000))
+    state = Swalbe.Sys(sys, "GPU")
+    Swalbe.equilibrium(state, sys)
+    Swalbe.time_loop(sys, state)
+"""
+from ._lib import DomainError, SwalbeError, device_count, lib
+from .core import (BGKandStream, DeviceArray, HipState, Sys, SysConst, Taumucs, cospi, equilibrium,
+                   fast_32, fast_93, filmpressure, force_sum, grad_f, h_grad_p, h_grad_p_raw,
+                   inclination, laplacian_f, moments, power_broad, slippage, thermal, time_loop,
+                   wetted)
+from .initialvalues import (normals, randinterface, random_film, rivulet, sinewave, singledroplet,
+                            torus, two_droplets_2d, uniforms)
+from .simulate import (run_dropletforced, run_dropletpatterned, run_dropletrelax, run_flat,
+                       run_random, run_rayleightaylor)
+
+lib()  # the HIP library is required: fail at import if it is missing
+
+__version__ = "1.0.0"

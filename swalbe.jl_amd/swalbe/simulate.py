@@ -1,0 +1,87 @@
+"""The 2D run_* drivers of src/simulate.jl:236-484 on a GPU-resident state."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import core
+from .core import Sys, SysConst, equilibrium, inclination, time_loop, wetted
+from .initialvalues import randinterface, sinewave, singledroplet
+
+
+def run_flat(sys: SysConst, device: str = "GPU", verbos: bool = True, T=np.float64):
+    """run_flat (src/simulate.jl:236-245): a flat film stays flat."""
+    print("Simulating a flat interface without driving forces (nothing should happen) in two dimensions")
+    state = Sys(sys, device, T=T)
+    state.height = 1.0
+    time_loop(sys, state, verbose=verbos)
+    return state.height
+
+
+def run_random(sys: SysConst, device: str = "GPU", h0: float = 1.0, eps: float = 0.01,
+               verbos: bool = True, seed: int = 0, T=np.float64):
+    """run_random (src/simulate.jl:286-294): a randomly undulated film."""
+    print("Simulating a random undulated interface in two dimensions")
+    state = Sys(sys, device, T=T)
+    h = np.ones((sys.Lx, sys.Ly), order="F")
+    randinterface(h, h0, eps, seed=seed)
+    state.height = h
+    equilibrium(state, sys)
+    time_loop(sys, state, 1 / 9, verbose=verbos)
+    return state.height
+
+
+def run_rayleightaylor(sys: SysConst, device: str = "GPU", kx: int = 15, ky: int = 18, h0: float = 1.0,
+                       eps: float = 0.001, verbos: bool = True, T=np.float64):
+    """run_rayleightaylor (src/simulate.jl:338-358): returns (height, Δh series)."""
+    print("Simulating the Rayleigh Taylor instability in two dimensions")
+    state = Sys(sys, device, T=T)
+    state.height = sinewave(sys.Lx, sys.Ly, h0, kx, ky, eps)
+    diff = []
+    equilibrium(state, sys)
+    time_loop(sys, state, diff, verbose=verbos)
+    return state.height, diff
+
+
+def run_dropletrelax(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, center=None,
+                     verbos: bool = True, T=np.float64):
+    """run_dropletrelax (src/simulate.jl:384-399): returns (height, wetted area series)."""
+    print("Simulating an out of equilibrium droplet in two dimensions")
+    center = center or (sys.Lx // 2, sys.Ly // 2)
+    state = Sys(sys, device, T=T)
+    state.height = singledroplet(np.ones((sys.Lx, sys.Ly), order="F"), radius, θ0, center)
+    equilibrium(state, sys)
+    area = []
+    time_loop(sys, state, wetted, area, verbose=verbos)
+    return state.height, area
+
+
+def run_dropletpatterned(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, center=None,
+                         θs=None, verbos: bool = True, T=np.float64):
+    """run_dropletpatterned (src/simulate.jl:422-439): contact-angle field θₛ."""
+    print("Simulating a droplet on a patterned substrate in two dimensions")
+    center = center or (sys.Lx // 2, sys.Ly // 2)
+    θs = np.full((sys.Lx, sys.Ly), 1 / 9) if θs is None else θs
+    state = Sys(sys, device, T=T)
+    state.height = singledroplet(np.ones((sys.Lx, sys.Ly), order="F"), radius, θ0, center)
+    equilibrium(state, sys)
+    time_loop(sys, state, θs, verbose=verbos)
+    return state.height
+
+
+def run_dropletforced(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, center=None, fx=0.0,
+                      fy=0.0, verbos: bool = True, T=np.float64):
+    """run_dropletforced (src/simulate.jl:462-484): a droplet under a body force;
+    returns (height, velx, vely)."""
+    bodyforce = [fx, fy]
+    print("Simulating a sliding droplet in two dimensions")
+    center = center or (sys.Lx // 2, sys.Ly // 2)
+    state = Sys(sys, device, T=T)
+    state.height = singledroplet(np.ones((sys.Lx, sys.Ly), order="F"), radius, θ0, center)
+    equilibrium(state, sys)
+    print("Starting the lattice Boltzmann time loop")
+    time_loop(sys, state, inclination, bodyforce, verbose=verbos)
+    return state.height, state.velx, state.vely
+
+
+__all__ = ["run_flat", "run_random", "run_rayleightaylor", "run_dropletrelax",
+           "run_dropletpatterned", "run_dropletforced", "core"]
