@@ -29,6 +29,7 @@ WORKLOADS = {
     # name: (Lx, Ly, dtype, thermal)
     "rupture4096_f32": (4096, 4096, "f32", True),
     "rupture4096_f64": (4096, 4096, "f64", True),
+    "flat4096_f32_nonoise": (4096, 4096, "f32", False),
     "droplet1024_f32": (1024, 1024, "f32", False),
 }
 
@@ -71,6 +72,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="rupture4096_f32", choices=sorted(WORKLOADS))
     ap.add_argument("--tile-y", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -93,7 +95,7 @@ def main():
     S.equilibrium(st, sysc)
     prm = param.cparams()
     o = _lib.StepOpts()
-    o.thermal, o.seed, o.tile_y = int(thermal), 7, a.tile_y
+    o.thermal, o.seed, o.tile_y, o.variant = int(thermal), 7, a.tile_y, a.variant
     ms = C.c_double()
     L = _lib.lib()
     # warmup
@@ -136,7 +138,7 @@ def main():
         "config": {"workload": f"{a.workload}: D2Q9 thin-film rupture with thermal noise, "
                                f"{Lx}x{Ly} periodic, kbt=1e-6, theta=1/9, n=9, m=3, gamma=0.01, tau=1",
                    "Lx": Lx, "Ly": Ly, "parallelism": "single GPU", "tile_y": a.tile_y or 16,
-                   "finite": ok},
+                   "variant": a.variant, "finite": ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_lup": bpl},

@@ -140,7 +140,9 @@ typedef struct swalbe_step_opts {
     double *dh_series;     /* host [nsteps]: max(h) - min(h) before each step (:56), or NULL */
     int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) (measures.jl:13-17), or NULL */
     double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
-    int32_t tile_y;        /* rows per workgroup tile: 0 = default */
+    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8, 16 or 32 */
+    int32_t variant;       /* FP32 kernel: 0 = default, 1 = collide at the source node,
+                              2 = collide at the destination node */
 } swalbe_step_opts;
 int swalbe_time_loop(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
                      const swalbe_step_opts *opts);

@@ -107,7 +107,7 @@ static const double COS_C[11] = {
 /* sin(pi x), cos(pi x): reduce to r in [-1/4, 1/4] by quarter turns. */
 void orc_sincospi(double x, double *s, double *c) {
     double ax = fabs(x);
-    if (ax >= 2.0) ax = fmod(ax, 2.0);
+    if (ax >= 2.0) ax = ax - 2.0 * floor(ax * 0.5);
     double k = floor(2.0 * ax + 0.5);
     double r = ax - 0.5 * k;
     double z = r * r;
@@ -135,11 +135,12 @@ double orc_cospi(double x) {
     return c;
 }
 
-/* Counter layout: {node, tag, step_lo, step_hi}, key {seed_lo, seed_hi}.
- * Two 53-bit uniforms in (0,1) from the four words, then Box-Muller. */
-static void philox_node(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag,
-                        uint32_t r[4]) {
-    uint32_t ctr[4] = {node, tag, (uint32_t)step, (uint32_t)(step >> 32)};
+/* Counter layout: {c0, tag, step_lo, step_hi}, key {seed_lo, seed_hi}.
+ * Normals: one call per node pair (c0 = node / 2); node parity b uses the
+ * 32-bit uniforms (w + 1/2) 2^-32 of words 2b and 2b+1, then Box-Muller.
+ * Uniforms: the node's own counter (c0 = node), 53 bits of words 0, 1. */
+static void philox_ctr(uint64_t seed, uint64_t step, uint32_t c0, uint32_t tag, uint32_t r[4]) {
+    uint32_t ctr[4] = {c0, tag, (uint32_t)step, (uint32_t)(step >> 32)};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     orc_philox4x32_10(ctr, key, r);
 }
@@ -149,12 +150,15 @@ static inline double u53(uint32_t hi, uint32_t lo) {
     return ((double)v + 0.5) * 1.1102230246251565e-16; /* 2^-53 */
 }
 
+static inline double u32(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
+
 void orc_normal_pair(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag, double *z0,
                      double *z1) {
     uint32_t r[4];
-    philox_node(seed, step, node, tag, r);
-    double u1 = u53(r[0], r[1]);
-    double u2 = u53(r[2], r[3]);
+    philox_ctr(seed, step, node >> 1, tag, r);
+    int b = (int)(node & 1u);
+    double u1 = u32(r[2 * b]);
+    double u2 = u32(r[2 * b + 1]);
     double rad = sqrt(-2.0 * orc_log(u1));
     double s, c;
     orc_sincospi(2.0 * u2, &s, &c);
@@ -164,7 +168,7 @@ void orc_normal_pair(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag, 
 
 double orc_uniform(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag) {
     uint32_t r[4];
-    philox_node(seed, step, node, tag, r);
+    philox_ctr(seed, step, node, tag, r);
     return u53(r[0], r[1]);
 }
 

@@ -152,6 +152,29 @@ int fill_field(swalbe_state* st, int field, double value) {
   return SWALBE_OK;
 }
 
+// κ(θ) field for θ-field runs, recomputed when θ or (n, m, hmin) change.
+int ensure_kappa(swalbe_state* st, int n, int m, double hmin) {
+  Film& F = st->film;
+  if (!F.theta) {
+    set_error("no θ field uploaded (SWALBE_THETA)");
+    return SWALBE_EINVAL;
+  }
+  if (!F.kappa)
+    if (int rc = alloc_field(&F.kappa, F.esz * F.N)) return rc;
+  if (F.kappa_valid && F.kappa_n == n && F.kappa_m == m && F.kappa_hmin == hmin) return SWALBE_OK;
+  if (F.dtype == SWALBE_F64)
+    launch_kappa_field<double>((double*)F.kappa, (const double*)F.theta, n, m, hmin, F.N, st->stream);
+  else
+    launch_kappa_field<float>((float*)F.kappa, (const float*)F.theta, n, m, hmin, F.N, st->stream);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "kappa field");
+  F.kappa_valid = true;
+  F.kappa_n = n;
+  F.kappa_m = m;
+  F.kappa_hmin = hmin;
+  return SWALBE_OK;
+}
+
 int set_device(swalbe_state* st) {
   hipError_t e = hipSetDevice(st->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
@@ -249,7 +272,8 @@ int swalbe_destroy(swalbe_state* st) {
   Film& F = st->film;
   void* ptrs[] = {F.f[0],  F.f[1],  F.feq,   F.hbuf[0], F.hbuf[1], F.uxbuf[0], F.uxbuf[1],
                   F.uybuf[0], F.uybuf[1], F.vsq, F.pressure, F.Fx, F.Fy, F.slipx, F.slipy,
-                  F.hgpx,  F.hgpy,  F.kbtx,  F.kbty,    F.theta,   st->partials, st->red_out};
+                  F.hgpx,  F.hgpy,  F.kbtx,  F.kbty,    F.theta,   F.kappa, st->partials,
+                  st->red_out};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (st->stream) hipStreamDestroy(st->stream);
@@ -275,6 +299,7 @@ int swalbe_upload(swalbe_state* st, int32_t field, const void* host, size_t byte
   }
   if (field == SWALBE_THETA && !F.theta)
     if (int rc = alloc_field(&F.theta, F.esz * F.N)) return rc;
+  if (field == SWALBE_THETA) F.kappa_valid = false;
   if (field == SWALBE_FOUT || field == SWALBE_FTEMP) unalias_for_write(F, field);
   int64_t c2;
   void* dst = field_ptr(st, field, &c2);
@@ -334,6 +359,7 @@ int swalbe_fill(swalbe_state* st, int32_t field, double value) {
   Film& F = st->film;
   if (field == SWALBE_THETA && !F.theta)
     if (int rc = alloc_field(&F.theta, F.esz * F.N)) return rc;
+  if (field == SWALBE_THETA) F.kappa_valid = false;
   if (field == SWALBE_FOUT || field == SWALBE_FTEMP) unalias_for_write(F, field);
   return F.dtype == SWALBE_F64 ? fill_field<double>(st, field, value)
                                : fill_field<float>(st, field, value);
@@ -450,13 +476,11 @@ int swalbe_filmpressure(swalbe_state* st, double gamma, double theta, int32_t n,
     set_error("filmpressure: negative exponent");
     return SWALBE_EINVAL;
   }
-  if (use_theta_field && !F.theta) {
-    set_error("filmpressure: no θ field uploaded (SWALBE_THETA)");
-    return SWALBE_EINVAL;
-  }
+  if (use_theta_field)
+    if (int rc = ensure_kappa(st, n, m, hmin)) return rc;
   const double kap = kappa(theta, n, m, hmin);
-  SW_BOTH(st, launch_filmpressure<T>((T*)F.pressure, (const T*)F.height(), gamma, kap,
-                                     use_theta_field ? (const T*)F.theta : nullptr, n, m, hmin,
+  SW_BOTH(st, launch_filmpressure<T>((T*)F.pressure, (const T*)F.height(), gamma, kap, nullptr,
+                                     use_theta_field ? (const T*)F.kappa : nullptr, n, m, hmin,
                                      hcrit, mode, F.Lx, F.Ly, F.enc, st->stream));
   SW_HIP(hipGetLastError());
   return SWALBE_OK;
@@ -623,7 +647,8 @@ StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, i
 }
 
 template <typename T>
-void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64_t t, int tile_y) {
+void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64_t t, int tile_y,
+                  int variant) {
   Film& F = st->film;
   const int cur = F.hu_cur;
   const int fdst = F.fout_buf != F.ftemp_buf ? F.fout_buf : 1 - F.ftemp_buf;
@@ -637,8 +662,11 @@ void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64
   io.ux_out = (T*)F.uxbuf[1 - cur];
   io.uy_out = (T*)F.uybuf[1 - cur];
   io.p_out = (T*)F.pressure;
-  io.theta = (const T*)F.theta;
-  launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.enc, tile_y, st->stream);
+  io.kappa = sp.use_theta ? (const T*)F.kappa : nullptr;
+  if constexpr (sizeof(T) == 8)
+    launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.enc, tile_y, st->stream);
+  else
+    launch_step_f32(io, sp, ser, t, F.Lx, F.Ly, F.enc, tile_y, variant, st->stream);
   F.fout_buf = F.ftemp_buf = fdst;
   F.hu_cur = 1 - cur;
 }
@@ -680,17 +708,19 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
   Film& F = st->film;
   swalbe_step_opts zero{};
   if (!o) o = &zero;
-  if (o->use_theta_field && !F.theta) {
-    set_error("swalbe_time_loop: no θ field uploaded (SWALBE_THETA)");
-    return SWALBE_EINVAL;
-  }
+  if (o->use_theta_field)
+    if (int rc = ensure_kappa(st, p->n, p->m, p->hmin)) return rc;
   if (p->n < 0 || p->m < 0) {
     set_error("swalbe_time_loop: negative exponent");
     return SWALBE_EINVAL;
   }
-  const int tile_y = o->tile_y > 0 ? o->tile_y : (F.dtype == SWALBE_F32 ? 16 : 16);
+  const int tile_y = o->tile_y > 0 ? o->tile_y : 16;
   if (tile_y != 8 && tile_y != 16 && tile_y != 32) {
     set_error("swalbe_time_loop: tile_y must be 8, 16 or 32");
+    return SWALBE_EINVAL;
+  }
+  if (o->variant < 0 || o->variant > 2) {
+    set_error("swalbe_time_loop: variant must be 0, 1 or 2");
     return SWALBE_EINVAL;
   }
   const StepParams sp = make_step_params(p, o, 0);
@@ -726,8 +756,8 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
       ++km;
     }
     StepSeries ser{want_dh ? lb.keys + 2 * (it - 1) : nullptr, want_wet ? lb.wet + (it - 1) : nullptr};
-    if (F.dtype == SWALBE_F64) enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y);
-    else enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y);
+    if (F.dtype == SWALBE_F64) enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y, o->variant);
+    else enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y, o->variant);
   }
   if (ms_out) {
     SW_HIP(hipEventRecord(e1, st->stream));
@@ -855,8 +885,8 @@ int swalbe_filmpressure_ptr(int32_t dtype, void* p, const void* h, double gamma,
   }
   const double kap = kappa(theta, n, m, hmin);
   SW_RAW_DISPATCH(dtype, launch_filmpressure<T>((T*)p, (const T*)h, gamma, kap,
-                                                (const T*)theta_field, n, m, hmin, hcrit, mode, Lx,
-                                                Ly, zero_enc(), (hipStream_t)stream));
+                                                (const T*)theta_field, nullptr, n, m, hmin, hcrit,
+                                                mode, Lx, Ly, zero_enc(), (hipStream_t)stream));
   SW_HIP(hipGetLastError());
   return SWALBE_OK;
 }

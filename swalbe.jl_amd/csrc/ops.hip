@@ -170,8 +170,8 @@ __device__ __forceinline__ T powlaw(T a, int n, int m, int mode) {
 
 template <typename T>
 __global__ void k_filmpressure(T* __restrict__ p, const T* __restrict__ h, T gamma, T kap,
-                               const T* __restrict__ theta, int n, int m, T hmin, T hcrit,
-                               int mode, int Lx, int Ly, Enc e) {
+                               const T* __restrict__ theta, const T* __restrict__ kapf, int n,
+                               int m, T hmin, T hcrit, int mode, int Lx, int Ly, Enc e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= Lx || j >= Ly) return;
@@ -184,7 +184,8 @@ __global__ void k_filmpressure(T* __restrict__ p, const T* __restrict__ h, T gam
                    H(im, jp), H(i, j));
   const T hh = dec_h(H(i, j), e);
   T k = kap;
-  if (theta) k = (T)kappa((double)theta[x], n, m, (double)hmin);
+  if (kapf) k = kapf[x];
+  else if (theta) k = (T)kappa((double)theta[x], n, m, (double)hmin);
   const T a = hmin / (hh + hcrit);
   const T p0 = -gamma * (k * powlaw(a, n, m, mode));
   p[x] = p0 - gamma * L;
@@ -299,6 +300,14 @@ __global__ void k_force_sum(T* __restrict__ Fx, T* __restrict__ Fy, const T* __r
 }
 
 template <typename T>
+__global__ void k_kappa(T* __restrict__ kap, const T* __restrict__ theta, int n, int m,
+                        double hmin, int64_t N) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < N;
+       x += (int64_t)gridDim.x * blockDim.x)
+    kap[x] = (T)kappa((double)theta[x], n, m, hmin);
+}
+
+template <typename T>
 __global__ void k_fill(T* __restrict__ p, T v, int64_t N) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < N;
        x += (int64_t)gridDim.x * blockDim.x)
@@ -346,10 +355,15 @@ void launch_moments(T* h, T* ux, T* uy, const T* f, int Lx, int Ly, Enc enc, hip
 }
 template <typename T>
 void launch_filmpressure(T* p, const T* h, double gamma, double kappa_scalar, const T* theta,
-                         int n, int m, double hmin, double hcrit, int powmode, int Lx, int Ly,
-                         Enc enc, hipStream_t s) {
+                         const T* kapf, int n, int m, double hmin, double hcrit, int powmode,
+                         int Lx, int Ly, Enc enc, hipStream_t s) {
   hipLaunchKernelGGL(k_filmpressure<T>, grid2d(Lx, Ly), dim3(kBX, kBY), 0, s, p, h, (T)gamma,
-                     (T)kappa_scalar, theta, n, m, (T)hmin, (T)hcrit, powmode, Lx, Ly, enc);
+                     (T)kappa_scalar, theta, kapf, n, m, (T)hmin, (T)hcrit, powmode, Lx, Ly, enc);
+}
+template <typename T>
+void launch_kappa_field(T* kap, const T* theta, int n, int m, double hmin, int64_t N,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_kappa<T>, grid1d(N), dim3(kBlock1D), 0, s, kap, theta, n, m, hmin, N);
 }
 template <typename T>
 void launch_grad(T* ox, T* oy, const T* f, const T* a, const T* h, int Lx, int Ly, Enc enc,
@@ -408,8 +422,9 @@ void launch_init_random(T* h, int dist, uint64_t seed, double base, double amp, 
   template void launch_bgk_and_stream<T>(T*, const T*, const T*, const T*, const T*, double, int, \
                                          int, Enc, hipStream_t);                                 \
   template void launch_moments<T>(T*, T*, T*, const T*, int, int, Enc, hipStream_t);             \
-  template void launch_filmpressure<T>(T*, const T*, double, double, const T*, int, int, double,  \
-                                       double, int, int, int, Enc, hipStream_t);                 \
+  template void launch_filmpressure<T>(T*, const T*, double, double, const T*, const T*, int, int, \
+                                       double, double, int, int, int, Enc, hipStream_t);         \
+  template void launch_kappa_field<T>(T*, const T*, int, int, double, int64_t, hipStream_t);      \
   template void launch_grad<T>(T*, T*, const T*, const T*, const T*, int, int, Enc, hipStream_t); \
   template void launch_laplacian<T>(T*, const T*, double, int, int, hipStream_t);                 \
   template void launch_slippage<T>(T*, T*, const T*, const T*, const T*, double, double, int, int, \

@@ -159,11 +159,11 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
                 T(10.0 / 3.0) * hc[0];
     const T hh = DEV ? href + hc[0] : hc[0];
     T kap = (T)sp.kappa;
-    if (sp.use_theta) {
+    if (io.kappa) {
       int gx = x0 - 2 + c, gy = y0 - 2 + r;
       gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
       gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
-      kap = (T)kappa((double)io.theta[gx + (int64_t)Lx * gy], sp.n, sp.m, sp.hmin);
+      kap = io.kappa[gx + (int64_t)Lx * gy];
     }
     const T a = hmin / (hh + hcrit);
     const T p0 = -gamma * (kap * powlaw(a, sp.n, sp.m, sp.powmode));
@@ -174,8 +174,8 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   // ---- phase C: forces on the 1-halo (src/forcing.jl:42-46, 168-187, 297-311, 363-368)
   {
     const T c6mu = (T)sp.c6mu, c6d = (T)sp.c6d, c3dd = (T)sp.c3dd;
-    for (int idx = tid; idx < UR * UC; idx += NT) {
-      const int r = idx / UC, c = idx - r * UC;
+    auto fnode = [&](int r, int c, double z0, double z1) {
+      const int idx = r * UC + c;
       const T* pc = &sh_p[(r + 1) * PC + (c + 1)];
       const T fip = pc[-1], fim = pc[1], fjp = pc[-PC], fjm = pc[PC];
       const T fipjp = pc[-PC - 1], fimjp = pc[-PC + 1], fimjm = pc[PC + 1], fipjm = pc[PC - 1];
@@ -190,11 +190,6 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
       T Fx = -hgx - sx;
       T Fy = -hgy - sy;
       if constexpr (THERMAL) {
-        int gxx = x0 - 1 + c, gyy = y0 - 1 + r;
-        gxx = small ? wrapm(gxx, Lx) : wrap1(gxx, Lx);
-        gyy = small ? wrapm(gyy, Ly) : wrap1(gyy, Ly);
-        double z0, z1;
-        normal_pair(sp.seed, step, (uint32_t)(gxx + (int64_t)Lx * gyy), kTagThermal, &z0, &z1);
         const T amp = sqrt((T)sp.kb26 * hh /
                            (((T(2) * hh) * hh + (T(6) * hh) * (T)sp.delta) + (T)sp.d3dd));
         Fx = Fx - (T)z0 * amp;
@@ -206,6 +201,38 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
       }
       sh_fx[idx] = Fx;
       sh_fy[idx] = Fy;
+    };
+    if (!small && (Lx & 1) == 0) {
+      // one Philox call per pair of x-adjacent nodes: 34 pair columns cover
+      // x0-2 .. x0+65, tile column c = 2 pc - 1 + b
+      constexpr int NPC = (TX + 4) / 2;
+      for (int t = tid; t < UR * NPC; t += NT) {
+        const int r = t / NPC, pc = t - r * NPC;
+        double z[4] = {0.0, 0.0, 0.0, 0.0};
+        if constexpr (THERMAL) {
+          const int gxx = wrap1(x0 - 2 + 2 * pc, Lx);
+          const int64_t node = gxx + (int64_t)Lx * wrap1(y0 - 1 + r, Ly);
+          uint32_t w[4];
+          philox_ctr(sp.seed, step, (uint32_t)(node >> 1), kTagThermal, w);
+          box_muller(u32(w[0]), u32(w[1]), &z[0], &z[1]);
+          box_muller(u32(w[2]), u32(w[3]), &z[2], &z[3]);
+        }
+        const int c0 = 2 * pc - 1;
+        if (c0 >= 0) fnode(r, c0, z[0], z[1]);
+        if (c0 + 1 < UC) fnode(r, c0 + 1, z[2], z[3]);
+      }
+    } else {
+      for (int idx = tid; idx < UR * UC; idx += NT) {
+        const int r = idx / UC, c = idx - r * UC;
+        double z0 = 0.0, z1 = 0.0;
+        if constexpr (THERMAL) {
+          int gxx = x0 - 1 + c, gyy = y0 - 1 + r;
+          gxx = small ? wrapm(gxx, Lx) : wrap1(gxx, Lx);
+          gyy = small ? wrapm(gyy, Ly) : wrap1(gyy, Ly);
+          normal_pair(sp.seed, step, (uint32_t)(gxx + (int64_t)Lx * gyy), kTagThermal, &z0, &z1);
+        }
+        fnode(r, c, z0, z1);
+      }
     }
   }
   __syncthreads();
@@ -389,12 +416,6 @@ void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint
                  int Lx, int Ly, Enc enc, int tile_y, hipStream_t s) {
   switch (tile_y) {
     case 8: launch_ty<T, 8>(io, sp, ser, step, Lx, Ly, enc, s); break;
-    case 32:
-      if constexpr (sizeof(T) == 4) {
-        launch_ty<T, 32>(io, sp, ser, step, Lx, Ly, enc, s);
-        break;
-      }
-      [[fallthrough]];
     default: launch_ty<T, 16>(io, sp, ser, step, Lx, Ly, enc, s); break;
   }
 }
@@ -412,8 +433,6 @@ void launch_reduce_finish(const double* partials, int nparts, double* out, hipSt
 
 template void launch_step<double>(const StepIO<double>&, const StepParams&, StepSeries, uint64_t,
                                   int, int, Enc, int, hipStream_t);
-template void launch_step<float>(const StepIO<float>&, const StepParams&, StepSeries, uint64_t,
-                                 int, int, Enc, int, hipStream_t);
 template void launch_reduce<double>(const double*, int64_t, double, double, double*, int,
                                     hipStream_t);
 template void launch_reduce<float>(const float*, int64_t, double, double, double*, int,

@@ -50,6 +50,10 @@ struct Film {
   void* kbtx = nullptr;
   void* kbty = nullptr;
   void* theta = nullptr;              // θ field, allocated on first upload
+  void* kappa = nullptr;              // κ(θ) field derived from theta
+  bool kappa_valid = false;
+  int kappa_n = 0, kappa_m = 0;
+  double kappa_hmin = 0.0;
   void* height() const { return hbuf[hu_cur]; }
   void* velx() const { return uxbuf[hu_cur]; }
   void* vely() const { return uybuf[hu_cur]; }
@@ -68,8 +72,8 @@ template <typename T>
 void launch_moments(T* h, T* ux, T* uy, const T* f, int Lx, int Ly, Enc enc, hipStream_t s);
 template <typename T>
 void launch_filmpressure(T* p, const T* h, double gamma, double kappa_scalar, const T* theta,
-                         int n, int m, double hmin, double hcrit, int powmode, int Lx, int Ly,
-                         Enc enc, hipStream_t s);
+                         const T* kapf, int n, int m, double hmin, double hcrit, int powmode,
+                         int Lx, int Ly, Enc enc, hipStream_t s);
 template <typename T>
 void launch_grad(T* ox, T* oy, const T* f, const T* a, const T* h, int Lx, int Ly, Enc enc,
                  hipStream_t s);
@@ -92,6 +96,10 @@ void launch_fill(T* p, double v, int64_t N, hipStream_t s);
 template <typename T>
 void launch_init_random(T* h, int dist, uint64_t seed, double base, double amp, double href,
                         int Lx, int Ly, hipStream_t s);
+// κ(θ) per node from a θ field (same arithmetic as the scalar κ).
+template <typename T>
+void launch_kappa_field(T* kap, const T* theta, int n, int m, double hmin, int64_t N,
+                        hipStream_t s);
 
 // ---------------------------------------------------------------- fused step
 struct StepParams {
@@ -126,14 +134,15 @@ struct StepIO {
   T* ux_out;
   T* uy_out;
   T* p_out;
-  const T* theta;
-  // optional materialised intermediates (last step of a loop)
-  T *feq, *vsq, *Fx, *Fy, *slipx, *slipy, *hgpx, *hgpy, *kbtx, *kbty;
+  const T* kappa;   // κ(θ) field (θ-field runs), or null for the scalar κ
 };
 
 template <typename T>
 void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step,
                  int Lx, int Ly, Enc enc, int tile_y, hipStream_t s);
+// FP32 films: variant 0/1 collide at the source node, 2 at the destination.
+void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
+                     int Lx, int Ly, Enc enc, int tile_y, int variant, hipStream_t s);
 
 // Deterministic FP64 reductions of a field: sum, max, min, count(> thresh).
 template <typename T>

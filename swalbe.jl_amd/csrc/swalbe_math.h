@@ -49,9 +49,9 @@ constexpr uint32_t kTagThermal = 0u;
 constexpr uint32_t kTagInitNormal = 1u;
 constexpr uint32_t kTagInitUniform = 2u;
 
-SW_HD void philox_node(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag,
-                       uint32_t r[4]) {
-  r[0] = node;
+// Philox words for counter {c0, tag, step_lo, step_hi}, key {seed_lo, seed_hi}.
+SW_HD void philox_ctr(uint64_t seed, uint64_t step, uint32_t c0, uint32_t tag, uint32_t r[4]) {
+  r[0] = c0;
   r[1] = tag;
   r[2] = (uint32_t)step;
   r[3] = (uint32_t)(step >> 32);
@@ -63,6 +63,9 @@ SW_HD double u53(uint32_t hi, uint32_t lo) {
   const uint64_t v = ((((uint64_t)hi) << 32) | lo) >> 11;
   return ((double)v + 0.5) * 1.1102230246251565e-16;
 }
+
+// 32-bit uniform in (0, 1): (w + 1/2) 2^-32 (exact in double).
+SW_HD double u32(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
 
 // ---------------------------------------------------------------- log
 // x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m-1)/(m+1),
@@ -153,13 +156,9 @@ SW_HD double cospi(double x) {
   return c;
 }
 
-// Two standard normals for (seed, step, node, tag) by Box-Muller.
-SW_HD void normal_pair(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag, double* z0,
-                       double* z1) {
-  uint32_t r[4];
-  philox_node(seed, step, node, tag, r);
-  const double u1 = u53(r[0], r[1]);
-  const double u2 = u53(r[2], r[3]);
+// Box-Muller on two uniforms: z0 = r cos(2 pi u2), z1 = r sin(2 pi u2),
+// r = sqrt(-2 ln u1).
+SW_HD void box_muller(double u1, double u2, double* z0, double* z1) {
   const double rad = sqrt(-2.0 * log_pos(u1));
   double s, c;
   sincospi(2.0 * u2, &s, &c);
@@ -167,10 +166,33 @@ SW_HD void normal_pair(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag
   *z1 = rad * s;
 }
 
+// Two standard normals for (seed, step, node, tag).  One Philox call serves
+// the node pair {2q, 2q+1}, q = node / 2: node parity b takes words 2b, 2b+1.
+SW_HD void normal_pair(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag, double* z0,
+                       double* z1) {
+  uint32_t r[4];
+  philox_ctr(seed, step, node >> 1, tag, r);
+  const int b = (int)(node & 1u);
+  box_muller(u32(r[2 * b]), u32(r[2 * b + 1]), z0, z1);
+}
+
+// Uniform of node (seed, step, node, tag): 53 bits of the node's own counter.
 SW_HD double uniform(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag) {
   uint32_t r[4];
-  philox_node(seed, step, node, tag, r);
+  philox_ctr(seed, step, node, tag, r);
   return u53(r[0], r[1]);
+}
+
+// FP32 Box-Muller on the same two words with the gfx950 transcendental
+// instructions (v_log_f32 = log2, v_sin/v_cos_f32 take revolutions): the
+// uniform is the top 24 bits of w, so the result is the FP64 normal to
+// within FP32 rounding.
+__device__ __forceinline__ void box_muller_f32(uint32_t w1, uint32_t w2, float* z0, float* z1) {
+  const float u1 = ((float)(w1 >> 8) + 0.5f) * 5.9604644775390625e-08f;
+  const float u2 = ((float)(w2 >> 8) + 0.5f) * 5.9604644775390625e-08f;
+  const float rad = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+  *z0 = rad * __builtin_amdgcn_cosf(u2);
+  *z1 = rad * __builtin_amdgcn_sinf(u2);
 }
 
 // Disjoining-pressure constant κ(θ) = (1-cos πθ)(n-1)(m-1)/((n-m) hmin),

@@ -221,29 +221,92 @@ def test_generic_callback_path(S):
         assert np.array_equal(a.get(name), b.get(name)), name
 
 
-@pytest.mark.parametrize("shape,tile", [((256, 96), 8), ((256, 96), 16), ((200, 130), 32)])
-def test_fp32_deviation_mode_tolerance(S, shape, tile):
+FP32_CASES = [
+    # (shape, tile_y, variant, features)
+    ((256, 96), 8, 1, "thermal"),
+    ((256, 96), 16, 1, "thermal"),
+    ((256, 96), 16, 2, "thermal"),
+    ((200, 130), 32, 2, "thermal"),
+    ((200, 130), 8, 2, "thermal"),
+    ((199, 61), 16, 1, "thermal"),      # odd Lx: per-node noise path
+    ((40, 20), 16, 1, "thermal"),       # small domain: modulo wrap path
+    ((192, 64), 16, 1, "theta"),
+    ((192, 64), 16, 2, "theta"),
+    ((192, 64), 16, 1, "tau"),
+    ((192, 64), 16, 2, "tau"),
+    ((192, 64), 16, 1, "body"),
+    ((192, 64), 16, 2, "body"),
+]
+
+
+@pytest.mark.parametrize("shape,tile,variant,feat", FP32_CASES,
+                         ids=[f"{a}x{b}-t{t}-v{v}-{f}" for (a, b), t, v, f in FP32_CASES])
+def test_fp32_deviation_mode_tolerance(S, shape, tile, variant, feat):
     """FP32 storage/compute with height and f stored as deviations from href:
-    rel-L2 of h, u vs the FP64 oracle after 400 steps, with thermal noise."""
-    print(f"[parity] fp32 {shape} tile {tile}", flush=True)
+    rel-L2 of h vs the FP64 oracle after 400 steps <= 1e-5, and the film's
+    deviation from its mean (the dynamics) within 1e-3."""
+    print(f"[parity] fp32 {shape} tile {tile} variant {variant} {feat}", flush=True)
     Lx, Ly = shape
     nsteps = 400
-    h0 = O.init_normal(Lx, Ly, 3, 1.0, 0.001)
-    prm = O.params(gamma=0.01, kbt=1e-6, tdump=100)
+    h0 = O.init_normal(Lx, Ly, 3, 1.0, 0.001 if feat != "theta" else 0.02)
+    tau = 0.8 if feat == "tau" else 1.0
+    theta = None
+    if feat == "theta":
+        theta = np.asfortranarray(np.random.default_rng(6).uniform(1 / 18, 1 / 6, shape))
+    body = (2e-5, -1e-5, 1.0) if feat == "body" else None
+    prm = O.params(gamma=0.01, kbt=1e-6, tdump=100, tau=tau)
     ost = _oracle_state(h0)
-    O.time_loop(ost, prm, nsteps, thermal=True, seed=42)
-    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=100, kbt=1e-6))
+    ost.ftemp[...] = ost.feq  # start from equilibrium so τ ≠ 1 conserves mass
+    O.time_loop(ost, prm, nsteps, thermal=True, seed=42, theta_field=theta, body=body)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=100, kbt=1e-6, τ=tau))
     st = S.Sys(sysc, "GPU", T=np.float32)
     st.set_reference_height(1.0)
     st.height = h0
     S.equilibrium(st, sysc)
-    S.time_loop(sysc, st, thermal=True, seed=42, tile_y=tile)
+    st.ftemp = st.get("feq", np.float64)
+    import ctypes as C
+    from swalbe import _lib
+    o = _lib.StepOpts()
+    o.thermal, o.seed, o.tile_y, o.variant = 1, 42, tile, variant
+    prmc = sysc.param.cparams()
+    if theta is not None:
+        st.theta = theta
+        o.use_theta_field = 1
+    if body is not None:
+        o.body, o.body_ax, o.body_ay, o.body_fac = 1, body[0], body[1], 1.0
+    _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(prmc), nsteps, C.byref(o)))
     h = st.get("height", np.float64)
     e = relL2(h, ost.height)
-    print(f"[parity] fp32 rel-L2 h = {e:.3e}", flush=True)
-    assert e <= 1e-5
-    # the dynamics: deviation from the mean also tracked
     dev = relL2(h - h.mean(), ost.height - ost.height.mean())
+    ux = st.get("velx", np.float64)
+    eu = relL2(ux, ost.velx)
+    print(f"[parity] fp32 rel-L2 h = {e:.3e}, dynamics {dev:.3e}, ux {eu:.3e}", flush=True)
+    assert e <= 1e-5
     assert dev <= 1e-3, dev
+    assert eu <= 1e-3, eu
     m = st.reduce()["sum"]
     assert abs(m - ost.height.sum()) <= 1e-6 * ost.height.sum()
+
+
+def test_fp32_noise_matches_fp64_normals(S):
+    """Strong noise (kbt = 1e-3) dominates the forcing, so the FP32 kernel's
+    hardware Box-Muller must draw the FP64 normals to FP32 precision."""
+    Lx, Ly, nsteps = 128, 64, 20
+    h0 = np.ones((Lx, Ly), order="F")
+    prm = O.params(kbt=1e-3, tdump=100)
+    ost = _oracle_state(h0)
+    O.time_loop(ost, prm, nsteps, thermal=True, seed=9)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=100, kbt=1e-3))
+    for variant in (1, 2):
+        st = S.Sys(sysc, "GPU", T=np.float32)
+        st.height = h0
+        S.equilibrium(st, sysc)
+        import ctypes as C
+        from swalbe import _lib
+        o = _lib.StepOpts()
+        o.thermal, o.seed, o.variant = 1, 9, variant
+        _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(sysc.param.cparams()), nsteps, C.byref(o)))
+        ux = st.get("velx", np.float64)
+        d = relL2(ux, ost.velx)
+        print(f"[parity] fp32 strong-noise ux rel-L2 = {d:.3e} (variant {variant})", flush=True)
+        assert d <= 1e-4
