@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--workload", default="rupture4096_f32", choices=sorted(WORKLOADS))
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--nontemporal", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -96,6 +97,7 @@ def main():
     prm = param.cparams()
     o = _lib.StepOpts()
     o.thermal, o.seed, o.tile_y, o.variant = int(thermal), 7, a.tile_y, a.variant
+    o.nontemporal = a.nontemporal
     ms = C.c_double()
     L = _lib.lib()
     # warmup
@@ -138,7 +140,8 @@ def main():
         "config": {"workload": f"{a.workload}: D2Q9 thin-film rupture with thermal noise, "
                                f"{Lx}x{Ly} periodic, kbt=1e-6, theta=1/9, n=9, m=3, gamma=0.01, tau=1",
                    "Lx": Lx, "Ly": Ly, "parallelism": "single GPU", "tile_y": a.tile_y or 16,
-                   "variant": a.variant, "finite": ok},
+                   "variant": a.variant, "nontemporal": a.nontemporal,
+                   "plane_pad": int(os.environ.get("SWALBE_PLANE_PAD", "0")), "finite": ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_lup": bpl},

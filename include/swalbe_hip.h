@@ -142,7 +142,9 @@ typedef struct swalbe_step_opts {
     double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
     int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8, 16 or 32 */
     int32_t variant;       /* FP32 kernel: 0 = default, 1 = collide at the source node,
-                              2 = collide at the destination node */
+                              2 = collide at the destination node, 3 / 4 = the same as
+                              1 / 2 on a persistent, prefetching grid */
+    int32_t nontemporal;   /* FP32 kernel: streaming (non-temporal) stores of the outputs */
 } swalbe_step_opts;
 int swalbe_time_loop(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
                      const swalbe_step_opts *opts);

@@ -3,6 +3,7 @@
 // A swalbe_state owns every field of one film on one device and one HIP
 // stream (default priority).  Calls on one state are not re-entrant;
 // distinct states may be driven from distinct threads.
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -144,7 +145,7 @@ int fill_field(swalbe_state* st, int field, double value) {
   }
   if (is_dist(field)) {
     for (int k = 0; k < kQ; ++k)
-      launch_fill<T>(p + k * F.N, value - enc_offset(F, field, k), F.N, st->stream);
+      launch_fill<T>(p + k * F.fstride, value - enc_offset(F, field, k), F.N, st->stream);
   } else {
     launch_fill<T>(p, value - enc_offset(F, field, 0), F.N, st->stream);
   }
@@ -173,6 +174,27 @@ int ensure_kappa(swalbe_state* st, int n, int m, double hmin) {
   F.kappa_m = m;
   F.kappa_hmin = hmin;
   return SWALBE_OK;
+}
+
+// host <-> device copies of a field in the state's element type; the
+// distribution planes sit fstride elements apart on the device.
+int copy_h2d(swalbe_state* st, int field, void* dst, const void* src) {
+  const Film& F = st->film;
+  hipError_t e;
+  if (is_dist(field))
+    e = hipMemcpy2D(dst, F.fstride * F.esz, src, F.N * F.esz, F.N * F.esz, kQ, hipMemcpyHostToDevice);
+  else
+    e = hipMemcpy(dst, src, F.N * F.esz, hipMemcpyHostToDevice);
+  return e == hipSuccess ? SWALBE_OK : hip_fail(e, "copy to device");
+}
+int copy_d2h(swalbe_state* st, int field, void* dst, const void* src) {
+  const Film& F = st->film;
+  hipError_t e;
+  if (is_dist(field))
+    e = hipMemcpy2D(dst, F.N * F.esz, src, F.fstride * F.esz, F.N * F.esz, kQ, hipMemcpyDeviceToHost);
+  else
+    e = hipMemcpy(dst, src, F.N * F.esz, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SWALBE_OK : hip_fail(e, "copy to host");
 }
 
 int set_device(swalbe_state* st) {
@@ -214,6 +236,13 @@ int swalbe_create(int32_t Lx, int32_t Ly, int32_t dtype, int32_t kind, swalbe_st
   F.dtype = dtype;
   F.esz = dtype == SWALBE_F64 ? 8 : 4;
   F.N = (int64_t)Lx * Ly;
+  // distribution planes may be staggered by a pad (elements, env SWALBE_PLANE_PAD)
+  {
+    int64_t pad = 0;
+    if (const char* e = getenv("SWALBE_PLANE_PAD")) pad = atoll(e);
+    if (pad < 0) pad = 0;
+    F.fstride = F.N + pad;
+  }
   F.href = dtype == SWALBE_F32 ? 1.0 : 0.0;
   F.g_enc = 0.0;
   F.enc = dtype == SWALBE_F32 ? make_enc(F.href, 0.0) : zero_enc();
@@ -222,7 +251,7 @@ int swalbe_create(int32_t Lx, int32_t Ly, int32_t dtype, int32_t kind, swalbe_st
   const size_t fb = F.esz * F.N;
   void** fields[] = {&F.f[0], &F.f[1], &F.feq};
   for (void** p : fields)
-    if (rc == SWALBE_OK) rc = alloc_field(p, 9 * fb);
+    if (rc == SWALBE_OK) rc = alloc_field(p, 9 * F.esz * F.fstride);
   void** scal[] = {&F.hbuf[0], &F.hbuf[1], &F.uxbuf[0], &F.uxbuf[1], &F.uybuf[0], &F.uybuf[1],
                    &F.vsq,     &F.pressure, &F.Fx,     &F.Fy,      &F.slipx,    &F.slipy,
                    &F.hgpx,    &F.hgpy,     &F.kbtx,   &F.kbty};
@@ -311,8 +340,7 @@ int swalbe_upload(swalbe_state* st, int32_t field, const void* host, size_t byte
     else ((float*)tmp.data())[i] = (float)(v - off);
   }
   SW_HIP(hipStreamSynchronize(st->stream));
-  SW_HIP(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
-  return SWALBE_OK;
+  return copy_h2d(st, field, dst, tmp.data());
 }
 
 int swalbe_download(swalbe_state* st, int32_t field, void* host, size_t bytes) {
@@ -338,7 +366,7 @@ int swalbe_download(swalbe_state* st, int32_t field, void* host, size_t bytes) {
   }
   std::vector<unsigned char> tmp((size_t)count * F.esz);
   SW_HIP(hipStreamSynchronize(st->stream));
-  SW_HIP(hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
+  if (int rc = copy_d2h(st, field, tmp.data(), src)) return rc;
   for (int64_t i = 0; i < count; ++i) {
     const double raw = F.esz == 8 ? ((double*)tmp.data())[i] : (double)((float*)tmp.data())[i];
     const double off = enc_offset(F, field, (int)(i / F.N));
@@ -402,7 +430,7 @@ int swalbe_set_reference_height(swalbe_state* st, double href, double g) {
     std::vector<unsigned char> tmp((size_t)count * 4);
     for (int64_t j = 0; j < count; ++j)
       ((float*)tmp.data())[j] = (float)(keep[i][j] - enc_offset(F, id, (int)(j / F.N)));
-    SW_HIP(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    if (int rc = copy_h2d(st, id, dst, tmp.data())) return rc;
   }
   return SWALBE_OK;
 }
@@ -426,7 +454,7 @@ int swalbe_equilibrium(swalbe_state* st, double g) {
   if (int rc = set_device(st)) return rc;
   Film& F = st->film;
   SW_BOTH(st, launch_equilibrium<T>((T*)F.feq, (const T*)F.height(), (const T*)F.velx(),
-                                    (const T*)F.vely(), (T*)F.vsq, g, F.Lx, F.Ly, F.enc,
+                                    (const T*)F.vely(), (T*)F.vsq, g, F.Lx, F.Ly, F.fstride, F.enc,
                                     st->stream));
   SW_HIP(hipGetLastError());
   return SWALBE_OK;
@@ -438,8 +466,8 @@ int swalbe_bgk_and_stream(swalbe_state* st, double tau) {
   Film& F = st->film;
   const int dst = F.fout_buf != F.ftemp_buf ? F.fout_buf : 1 - F.ftemp_buf;
   SW_BOTH(st, launch_bgk_and_stream<T>((T*)F.f[dst], (const T*)F.feq, (const T*)F.f[F.ftemp_buf],
-                                       (const T*)F.Fx, (const T*)F.Fy, tau, F.Lx, F.Ly, F.enc,
-                                       st->stream));
+                                       (const T*)F.Fx, (const T*)F.Fy, tau, F.Lx, F.Ly, F.fstride,
+                                       F.enc, st->stream));
   SW_HIP(hipGetLastError());
   F.fout_buf = F.ftemp_buf = dst;
   return SWALBE_OK;
@@ -450,7 +478,8 @@ int swalbe_moments(swalbe_state* st) {
   if (int rc = set_device(st)) return rc;
   Film& F = st->film;
   SW_BOTH(st, launch_moments<T>((T*)F.height(), (T*)F.velx(), (T*)F.vely(),
-                                (const T*)F.f[F.fout_buf], F.Lx, F.Ly, F.enc, st->stream));
+                                (const T*)F.f[F.fout_buf], F.Lx, F.Ly, F.fstride, F.enc,
+                                st->stream));
   SW_HIP(hipGetLastError());
   return SWALBE_OK;
 }
@@ -648,7 +677,7 @@ StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, i
 
 template <typename T>
 void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64_t t, int tile_y,
-                  int variant) {
+                  int variant, int nts) {
   Film& F = st->film;
   const int cur = F.hu_cur;
   const int fdst = F.fout_buf != F.ftemp_buf ? F.fout_buf : 1 - F.ftemp_buf;
@@ -664,9 +693,10 @@ void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64
   io.p_out = (T*)F.pressure;
   io.kappa = sp.use_theta ? (const T*)F.kappa : nullptr;
   if constexpr (sizeof(T) == 8)
-    launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.enc, tile_y, st->stream);
+    launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.fstride, F.enc, tile_y, st->stream);
   else
-    launch_step_f32(io, sp, ser, t, F.Lx, F.Ly, F.enc, tile_y, variant, st->stream);
+    launch_step_f32(io, sp, ser, t, F.Lx, F.Ly, F.fstride, F.enc, tile_y, variant, nts,
+                    st->stream);
   F.fout_buf = F.ftemp_buf = fdst;
   F.hu_cur = 1 - cur;
 }
@@ -694,7 +724,8 @@ void materialize_last(swalbe_state* st, const swalbe_params* p, const swalbe_ste
   if (o->body)
     launch_inclination<T>((T*)F.Fx, (T*)F.Fy, h, o->body_ax, o->body_ay, o->body_fac, F.Lx, F.Ly,
                           F.enc, st->stream);
-  launch_equilibrium<T>((T*)F.feq, h, ux, uy, (T*)F.vsq, p->g, F.Lx, F.Ly, F.enc, st->stream);
+  launch_equilibrium<T>((T*)F.feq, h, ux, uy, (T*)F.vsq, p->g, F.Lx, F.Ly, F.fstride, F.enc,
+                        st->stream);
 }
 
 int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swalbe_step_opts* o,
@@ -719,8 +750,8 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
     set_error("swalbe_time_loop: tile_y must be 8, 16 or 32");
     return SWALBE_EINVAL;
   }
-  if (o->variant < 0 || o->variant > 2) {
-    set_error("swalbe_time_loop: variant must be 0, 1 or 2");
+  if (o->variant < 0 || o->variant > 4) {
+    set_error("swalbe_time_loop: variant must be 0..4");
     return SWALBE_EINVAL;
   }
   const StepParams sp = make_step_params(p, o, 0);
@@ -756,8 +787,10 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
       ++km;
     }
     StepSeries ser{want_dh ? lb.keys + 2 * (it - 1) : nullptr, want_wet ? lb.wet + (it - 1) : nullptr};
-    if (F.dtype == SWALBE_F64) enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y, o->variant);
-    else enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y, o->variant);
+    if (F.dtype == SWALBE_F64)
+      enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal);
+    else
+      enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal);
   }
   if (ms_out) {
     SW_HIP(hipEventRecord(e1, st->stream));
@@ -843,7 +876,8 @@ int swalbe_equilibrium_ptr(int32_t dtype, void* feq, const void* h, const void* 
                            void* vsq, double g, int32_t Lx, int32_t Ly, void* stream) {
   if (int rc = check_raw(dtype, Lx, Ly, {feq, h, ux, uy, vsq})) return rc;
   SW_RAW_DISPATCH(dtype, launch_equilibrium<T>((T*)feq, (const T*)h, (const T*)ux, (const T*)uy,
-                                               (T*)vsq, g, Lx, Ly, zero_enc(), (hipStream_t)stream));
+                                               (T*)vsq, g, Lx, Ly, (int64_t)Lx * Ly, zero_enc(),
+                                               (hipStream_t)stream));
   SW_HIP(hipGetLastError());
   return SWALBE_OK;
 }
@@ -858,7 +892,7 @@ int swalbe_bgk_and_stream_ptr(int32_t dtype, void* fout, const void* feq, void* 
   }
   SW_RAW_DISPATCH(dtype, launch_bgk_and_stream<T>((T*)fout, (const T*)feq, (const T*)ftemp,
                                                   (const T*)Fx, (const T*)Fy, tau, Lx, Ly,
-                                                  zero_enc(), (hipStream_t)stream));
+                                                  (int64_t)Lx * Ly, zero_enc(), (hipStream_t)stream));
   SW_HIP(hipGetLastError());
   const size_t bytes = (size_t)9 * Lx * Ly * (dtype == SWALBE_F64 ? 8 : 4);
   SW_HIP(hipMemcpyAsync(ftemp, fout, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
@@ -869,7 +903,7 @@ int swalbe_moments_ptr(int32_t dtype, void* h, void* ux, void* uy, const void* f
                        int32_t Ly, void* stream) {
   if (int rc = check_raw(dtype, Lx, Ly, {h, ux, uy, fout})) return rc;
   SW_RAW_DISPATCH(dtype, launch_moments<T>((T*)h, (T*)ux, (T*)uy, (const T*)fout, Lx, Ly,
-                                           zero_enc(), (hipStream_t)stream));
+                                           (int64_t)Lx * Ly, zero_enc(), (hipStream_t)stream));
   SW_HIP(hipGetLastError());
   return SWALBE_OK;
 }

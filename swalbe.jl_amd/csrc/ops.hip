@@ -52,7 +52,7 @@ inline dim3 grid1d(int64_t N) {
 template <typename T>
 __global__ void k_equilibrium(T* __restrict__ feq, const T* __restrict__ h,
                               const T* __restrict__ ux, const T* __restrict__ uy,
-                              T* __restrict__ vsq, T g, int64_t N, Enc e) {
+                              T* __restrict__ vsq, T g, int64_t N, int64_t fs, Enc e) {
   const T g0 = T(1.5) * g;
   const T w1 = T(1.0 / 9.0), w5 = T(1.0 / 36.0);
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < N;
@@ -75,7 +75,7 @@ __global__ void k_equilibrium(T* __restrict__ feq, const T* __restrict__ h,
     a = u - v;
     f[8] = w5 * hh * (((g0 * hh + T(3) * a) + T(4.5) * (a * a)) - T(1.5) * usq);
 #pragma unroll
-    for (int k = 0; k < kQ; ++k) feq[k * N + x] = enc_f(f[k], k, e);
+    for (int k = 0; k < kQ; ++k) feq[k * fs + x] = enc_f(f[k], k, e);
   }
 }
 
@@ -102,22 +102,22 @@ __device__ __forceinline__ T force_term(int k, T Fx, T Fy) {
 template <typename T>
 __global__ void k_bgk_stream(T* __restrict__ fdst, const T* __restrict__ feq,
                              const T* __restrict__ ftemp, const T* __restrict__ Fx,
-                             const T* __restrict__ Fy, T omeg, T itau, int Lx, int Ly, Enc e) {
+                             const T* __restrict__ Fy, T omeg, T itau, int Lx, int Ly, int64_t fs,
+                             Enc e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= Lx || j >= Ly) return;
-  const int64_t N = (int64_t)Lx * Ly;
   const int64_t x = i + (int64_t)Lx * j;
 #pragma unroll
   for (int k = 0; k < kQ; ++k) {
     const int is = wrapi(i - cx(k), Lx), js = wrapi(j - cy(k), Ly);
     const int64_t y = is + (int64_t)Lx * js;
-    const T ft = dec_f(ftemp[k * N + y], k, e);
-    const T fe = dec_f(feq[k * N + y], k, e);
+    const T ft = dec_f(ftemp[k * fs + y], k, e);
+    const T fe = dec_f(feq[k * fs + y], k, e);
     T fo = omeg * ft + itau * fe;
     // (a + b) - c*F is written as (a + b) + (-(c*F)); IEEE negation is exact
     if (k > 0) fo = fo + force_term<T>(k, Fx[y], Fy[y]);
-    fdst[k * N + x] = enc_f(fo, k, e);
+    fdst[k * fs + x] = enc_f(fo, k, e);
   }
 }
 
@@ -125,12 +125,12 @@ __global__ void k_bgk_stream(T* __restrict__ fdst, const T* __restrict__ feq,
 // src/moments.jl:43-52: h = f0 + f1 + ... + f8 (sum!), u = Σ c f / h.
 template <typename T>
 __global__ void k_moments(T* __restrict__ h, T* __restrict__ ux, T* __restrict__ uy,
-                          const T* __restrict__ f, int64_t N, Enc e) {
+                          const T* __restrict__ f, int64_t N, int64_t fs, Enc e) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < N;
        x += (int64_t)gridDim.x * blockDim.x) {
     T v[kQ];
 #pragma unroll
-    for (int k = 0; k < kQ; ++k) v[k] = dec_f(f[k * N + x], k, e);
+    for (int k = 0; k < kQ; ++k) v[k] = dec_f(f[k * fs + x], k, e);
     T s = v[0];
 #pragma unroll
     for (int k = 1; k < kQ; ++k) s = s + v[k];
@@ -336,22 +336,23 @@ __global__ void k_init_random(T* __restrict__ h, int dist, uint64_t seed, double
 // ---------------------------------------------------------------- launchers
 template <typename T>
 void launch_equilibrium(T* feq, const T* h, const T* ux, const T* uy, T* vsq, double g, int Lx,
-                        int Ly, Enc enc, hipStream_t s) {
+                        int Ly, int64_t fs, Enc enc, hipStream_t s) {
   const int64_t N = (int64_t)Lx * Ly;
   hipLaunchKernelGGL(k_equilibrium<T>, grid1d(N), dim3(kBlock1D), 0, s, feq, h, ux, uy, vsq,
-                     (T)g, N, enc);
+                     (T)g, N, fs, enc);
 }
 template <typename T>
 void launch_bgk_and_stream(T* fdst, const T* feq, const T* ftemp, const T* Fx, const T* Fy,
-                           double tau, int Lx, int Ly, Enc enc, hipStream_t s) {
+                           double tau, int Lx, int Ly, int64_t fs, Enc enc, hipStream_t s) {
   const double omeg = 1.0 - 1.0 / tau, itau = 1.0 / tau;
   hipLaunchKernelGGL(k_bgk_stream<T>, grid2d(Lx, Ly), dim3(kBX, kBY), 0, s, fdst, feq, ftemp, Fx,
-                     Fy, (T)omeg, (T)itau, Lx, Ly, enc);
+                     Fy, (T)omeg, (T)itau, Lx, Ly, fs, enc);
 }
 template <typename T>
-void launch_moments(T* h, T* ux, T* uy, const T* f, int Lx, int Ly, Enc enc, hipStream_t s) {
+void launch_moments(T* h, T* ux, T* uy, const T* f, int Lx, int Ly, int64_t fs, Enc enc,
+                    hipStream_t s) {
   const int64_t N = (int64_t)Lx * Ly;
-  hipLaunchKernelGGL(k_moments<T>, grid1d(N), dim3(kBlock1D), 0, s, h, ux, uy, f, N, enc);
+  hipLaunchKernelGGL(k_moments<T>, grid1d(N), dim3(kBlock1D), 0, s, h, ux, uy, f, N, fs, enc);
 }
 template <typename T>
 void launch_filmpressure(T* p, const T* h, double gamma, double kappa_scalar, const T* theta,
@@ -418,10 +419,10 @@ void launch_init_random(T* h, int dist, uint64_t seed, double base, double amp, 
 
 #define SW_INSTANTIATE(T)                                                                        \
   template void launch_equilibrium<T>(T*, const T*, const T*, const T*, T*, double, int, int,   \
-                                      Enc, hipStream_t);                                         \
+                                      int64_t, Enc, hipStream_t);                                \
   template void launch_bgk_and_stream<T>(T*, const T*, const T*, const T*, const T*, double, int, \
-                                         int, Enc, hipStream_t);                                 \
-  template void launch_moments<T>(T*, T*, T*, const T*, int, int, Enc, hipStream_t);             \
+                                         int, int64_t, Enc, hipStream_t);                        \
+  template void launch_moments<T>(T*, T*, T*, const T*, int, int, int64_t, Enc, hipStream_t);    \
   template void launch_filmpressure<T>(T*, const T*, double, double, const T*, const T*, int, int, \
                                        double, double, int, int, int, Enc, hipStream_t);         \
   template void launch_kappa_field<T>(T*, const T*, int, int, double, int64_t, hipStream_t);      \

@@ -104,7 +104,7 @@ __device__ __forceinline__ T force_term(int k, T Fx, T Fy) {
 template <typename T, int TY, bool READ_FT, bool THERMAL>
 __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSeries ser,
                                              uint64_t step, int Lx, int Ly, int ntx, int ntiles,
-                                             int small, Enc enc) {
+                                             int small, int64_t FS, Enc enc) {
   constexpr bool DEV = sizeof(T) == 4;
   constexpr int HC = TX + 6, HR = TY + 6;  // h tile, 3-halo
   constexpr int PC = TX + 4, PR = TY + 4;  // p tile, 2-halo
@@ -267,11 +267,11 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
         int sx = gxo - cx(k), sy = gyo - cy(k);
         sx = small ? wrapm(sx, Lx) : wrap1(sx, Lx);
         sy = small ? wrapm(sy, Ly) : wrap1(sy, Ly);
-        f = omeg * io.ft[k * N + sx + (int64_t)Lx * sy] + f;
+        f = omeg * io.ft[k * FS + sx + (int64_t)Lx * sy] + f;
       }
       if (k > 0) f = f + force_term<T>(k, sh_fx[su], sh_fy[su]);
       fo[k] = f;
-      io.f_out[k * N + xo] = f;
+      io.f_out[k * FS + xo] = f;
     }
     T s = fo[0];
 #pragma unroll
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
 
 template <typename T, int TY>
 void launch_ty(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step, int Lx,
-               int Ly, Enc enc, hipStream_t s) {
+               int Ly, int64_t FS, Enc enc, hipStream_t s) {
   const int ntx = (Lx + TX - 1) / TX, nty = (Ly + TY - 1) / TY;
   const int ntiles = ntx * nty;
   const int small = (Lx < TX + 6 || Ly < TY + 6) ? 1 : 0;
@@ -330,17 +330,17 @@ void launch_ty(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64
   if (sp.read_ft) {
     if (sp.thermal)
       hipLaunchKernelGGL((k_step<T, TY, true, true>), grid, block, 0, s, io, sp, ser, step, Lx, Ly,
-                         ntx, ntiles, small, enc);
+                         ntx, ntiles, small, FS, enc);
     else
       hipLaunchKernelGGL((k_step<T, TY, true, false>), grid, block, 0, s, io, sp, ser, step, Lx,
-                         Ly, ntx, ntiles, small, enc);
+                         Ly, ntx, ntiles, small, FS, enc);
   } else {
     if (sp.thermal)
       hipLaunchKernelGGL((k_step<T, TY, false, true>), grid, block, 0, s, io, sp, ser, step, Lx,
-                         Ly, ntx, ntiles, small, enc);
+                         Ly, ntx, ntiles, small, FS, enc);
     else
       hipLaunchKernelGGL((k_step<T, TY, false, false>), grid, block, 0, s, io, sp, ser, step, Lx,
-                         Ly, ntx, ntiles, small, enc);
+                         Ly, ntx, ntiles, small, FS, enc);
   }
 }
 
@@ -413,10 +413,10 @@ __global__ void k_reduce_finish(const double* __restrict__ partials, int nparts,
 
 template <typename T>
 void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step,
-                 int Lx, int Ly, Enc enc, int tile_y, hipStream_t s) {
+                 int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, hipStream_t s) {
   switch (tile_y) {
-    case 8: launch_ty<T, 8>(io, sp, ser, step, Lx, Ly, enc, s); break;
-    default: launch_ty<T, 16>(io, sp, ser, step, Lx, Ly, enc, s); break;
+    case 8: launch_ty<T, 8>(io, sp, ser, step, Lx, Ly, fstride, enc, s); break;
+    default: launch_ty<T, 16>(io, sp, ser, step, Lx, Ly, fstride, enc, s); break;
   }
 }
 
@@ -432,7 +432,7 @@ void launch_reduce_finish(const double* partials, int nparts, double* out, hipSt
 }
 
 template void launch_step<double>(const StepIO<double>&, const StepParams&, StepSeries, uint64_t,
-                                  int, int, Enc, int, hipStream_t);
+                                  int, int, int64_t, Enc, int, hipStream_t);
 template void launch_reduce<double>(const double*, int64_t, double, double, double*, int,
                                     hipStream_t);
 template void launch_reduce<float>(const float*, int64_t, double, double, double*, int,

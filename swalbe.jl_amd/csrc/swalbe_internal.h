@@ -30,6 +30,7 @@ struct Film {
   int dtype = SWALBE_F64;
   size_t esz = 8;
   int64_t N = 0;
+  int64_t fstride = 0;                // elements between distribution planes (>= N)
   double href = 0.0, g_enc = 0.0;  // encoding parameters (FP32 only)
   Enc enc{};
   void* f[2] = {nullptr, nullptr};    // distribution ping-pong buffers
@@ -62,14 +63,16 @@ struct Film {
 // ---------------------------------------------------------------- per-op launchers
 // All arrays are device pointers of element type T (double or float); `enc`
 // decodes FP32 deviation storage (zero_enc() for plain arrays).
+// Distribution arrays hold plane k at offset k*fs (fs = Lx*Ly for caller arrays).
 template <typename T>
 void launch_equilibrium(T* feq, const T* h, const T* ux, const T* uy, T* vsq, double g, int Lx,
-                        int Ly, Enc enc, hipStream_t s);
+                        int Ly, int64_t fs, Enc enc, hipStream_t s);
 template <typename T>
 void launch_bgk_and_stream(T* fdst, const T* feq, const T* ftemp, const T* Fx, const T* Fy,
-                           double tau, int Lx, int Ly, Enc enc, hipStream_t s);
+                           double tau, int Lx, int Ly, int64_t fs, Enc enc, hipStream_t s);
 template <typename T>
-void launch_moments(T* h, T* ux, T* uy, const T* f, int Lx, int Ly, Enc enc, hipStream_t s);
+void launch_moments(T* h, T* ux, T* uy, const T* f, int Lx, int Ly, int64_t fs, Enc enc,
+                    hipStream_t s);
 template <typename T>
 void launch_filmpressure(T* p, const T* h, double gamma, double kappa_scalar, const T* theta,
                          const T* kapf, int n, int m, double hmin, double hcrit, int powmode,
@@ -139,10 +142,11 @@ struct StepIO {
 
 template <typename T>
 void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step,
-                 int Lx, int Ly, Enc enc, int tile_y, hipStream_t s);
+                 int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, hipStream_t s);
 // FP32 films: variant 0/1 collide at the source node, 2 at the destination.
 void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
-                     int Lx, int Ly, Enc enc, int tile_y, int variant, hipStream_t s);
+                     int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, int variant, int nts,
+                     hipStream_t s);
 
 // Deterministic FP64 reductions of a field: sum, max, min, count(> thresh).
 template <typename T>

@@ -48,7 +48,7 @@ class StepOpts(C.Structure):
         ("body_ax", C.c_double), ("body_ay", C.c_double), ("body_fac", C.c_double),
         ("wet_thresh", C.c_double), ("dh_series", C.POINTER(C.c_double)),
         ("wet_series", C.POINTER(C.c_int64)), ("mass_series", C.POINTER(C.c_double)),
-        ("tile_y", C.c_int32), ("variant", C.c_int32),
+        ("tile_y", C.c_int32), ("variant", C.c_int32), ("nontemporal", C.c_int32),
     ]
 
 

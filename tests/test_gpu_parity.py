@@ -226,8 +226,11 @@ FP32_CASES = [
     ((256, 96), 8, 1, "thermal"),
     ((256, 96), 16, 1, "thermal"),
     ((256, 96), 16, 2, "thermal"),
-    ((200, 130), 32, 2, "thermal"),
+    ((256, 96), 8, 3, "thermal"),
+    ((256, 96), 16, 4, "thermal"),
+    ((200, 130), 8, 4, "thermal"),
     ((200, 130), 8, 2, "thermal"),
+    ((200, 130), 16, 0, "thermal"),
     ((199, 61), 16, 1, "thermal"),      # odd Lx: per-node noise path
     ((40, 20), 16, 1, "thermal"),       # small domain: modulo wrap path
     ((192, 64), 16, 1, "theta"),
@@ -236,6 +239,9 @@ FP32_CASES = [
     ((192, 64), 16, 2, "tau"),
     ((192, 64), 16, 1, "body"),
     ((192, 64), 16, 2, "body"),
+    ((192, 64), 16, 3, "tau"),
+    ((192, 64), 8, 4, "tau"),
+    ((192, 64), 16, 4, "theta"),
 ]
 
 
@@ -310,3 +316,28 @@ def test_fp32_noise_matches_fp64_normals(S):
         d = relL2(ux, ost.velx)
         print(f"[parity] fp32 strong-noise ux rel-L2 = {d:.3e} (variant {variant})", flush=True)
         assert d <= 1e-4
+
+
+def test_plane_padding_keeps_fp64_bitexact(S, monkeypatch):
+    """Distribution planes staggered by a pad (SWALBE_PLANE_PAD) change only
+    the device layout: fields read back and the fused loop stay bit-exact."""
+    monkeypatch.setenv("SWALBE_PLANE_PAD", "37")
+    Lx, Ly, nsteps = 70, 45, 30
+    h0 = O.init_normal(Lx, Ly, 5, 1.0, 0.05)
+    prm = O.params(tau=0.8, kbt=1e-4, tdump=10)
+    ost = _oracle_state(h0)
+    O.time_loop(ost, prm, nsteps, thermal=True, seed=3)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=10, τ=0.8, kbt=1e-4))
+    st = S.Sys(sysc, "GPU")
+    st.height = h0
+    S.equilibrium(st, sysc)
+    S.time_loop(sysc, st, thermal=True, seed=3)
+    for name, ref in (("height", ost.height), ("fout", ost.fout), ("ftemp", ost.ftemp),
+                      ("feq", ost.feq), ("velx", ost.velx)):
+        assert np.array_equal(st.get(name), ref), name
+    f = np.asfortranarray(np.random.default_rng(1).normal(size=(Lx, Ly, 9)))
+    st.fout = f
+    assert np.array_equal(st.fout, f)
+    st32 = S.Sys(sysc, "GPU", T=np.float32)
+    st32.feq = f
+    assert np.allclose(st32.get("feq", np.float64), f, atol=1e-6)
