@@ -141,7 +141,7 @@ typedef struct swalbe_step_opts {
     int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) (measures.jl:13-17), or NULL */
     double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
     int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8, 16 or 32 */
-    int32_t variant;       /* FP32 kernel: 0 = default, 1 = collide at the source node,
+    int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = collide at the source node,
                               2 = collide at the destination node, 3 / 4 = the same as
                               1 / 2 on a persistent, prefetching grid */
     int32_t nontemporal;   /* FP32 kernel: streaming (non-temporal) stores of the outputs */

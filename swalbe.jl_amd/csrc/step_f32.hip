@@ -445,13 +445,14 @@ void launch_f32_cfg(const StepIO<float>& io, const StepParams& sp, StepSeries se
 
 }  // namespace
 
-// variant: 0 = default (persistent, collide at the destination), 1 = collide
-// at the source node, 2 = collide at the destination, 3 = persistent +
-// source, 4 = persistent + destination.  nts: non-temporal stores.
+// variant: 0 = default (collide at the destination, one tile per
+// workgroup), 1 = collide at the source node, 2 = collide at the destination,
+// 3 = persistent + source, 4 = persistent + destination.  nts: non-temporal
+// stores.
 void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
                      int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, int variant, int nts,
                      hipStream_t s) {
-  const int v = variant == 0 ? 4 : variant;
+  const int v = variant == 0 ? 2 : variant;
   const bool t8 = tile_y == 8;
   switch (v) {
     case 1:
