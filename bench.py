@@ -32,6 +32,8 @@ WORKLOADS = {
     "flat4096_f32_nonoise": (4096, 4096, "f32", False),
     "droplet1024_f32": (1024, 1024, "f32", False),
 }
+# many-film workloads (FP32 batched kernel)
+BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson")
 
 
 def bytes_per_lup(esz: int, tau_is_one: bool) -> int:
@@ -65,12 +67,119 @@ def cpu_baseline(seconds_target: float = 12.0):
                       "unfused reference-order sweeps (oracle)"}
 
 
+def bench_batch(a):
+    """256 independent droplet-coalescence films (Lx in {128..1024} step 64,
+    Ly = Lx/2, two tangent caps, per-film θ, γ, radius) stepped together by
+    the batched FP32 kernel; K timed batch steps, run as captured graphs."""
+    import swalbe as S
+    from swalbe import serve
+    reqs = serve.coalescence_sweep(256, seed=1, nsteps=a.warmup + a.steps)
+    cap = sum(S.film_bytes(r.Lx, r.Ly) for r in reqs)
+    b = S.Batch(cap, 256, graph_steps=a.graph_steps)
+    for r in reqs:
+        b.add(r.param, r.Lx, r.Ly, r.init, r.nsteps)
+    st = b.stats()
+    b.step(a.warmup)
+    b.sync()
+    w0 = time.perf_counter()
+    ms = b.step_timed(a.steps)
+    wall = time.perf_counter() - w0
+    N = st["nodes"]
+    glups = N * a.steps / (ms / 1e3) / 1e9
+    bpl = bytes_per_lup(4, True)
+    achieved = bpl * glups
+    import numpy as np
+    ok = all(bool(np.isfinite(b.get(s, "height")).all()) for s in (0, 127, 255))
+    b.close()
+    line = {
+        "metric": "GLUPS of 256 batched FP32 droplet-coalescence films (one launch per step)",
+        "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms / a.steps, 5),
+        "wall_ms_per_step": round(wall * 1e3 / a.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded two-cap films)",
+        "config": {"workload": "coalesce256_batch: 256 films, Lx uniform in {128..1024} step 64, "
+                               "Ly = Lx/2, two tangent caps R ~ U(0.15,0.3) Ly, theta/pi ~ "
+                               "U(1/18,1/6), gamma ~ U(0.005,0.02), tau=1",
+                   "films": st["films"], "nodes": N, "graph_steps": a.graph_steps,
+                   "arena_bytes": cap, "parallelism": "single GPU", "finite": ok},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_lup": bpl},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(a):
+    """Continuous-batching serving of coalescence solve requests: Poisson
+    arrivals, Lx in [128, 2048] (multiples of 64), Ly = Lx/2 or Lx,
+    T in {10^3, 10^4, 5 10^4}; admission by HBM footprint.  Reports solves/s
+    and p50/p99 latency (arrival -> height on the host) beside the same
+    requests solved one at a time (fused single-film kernel), and the rel-L2
+    of a 5 % sample against FP64 solves."""
+    import numpy as np
+    import swalbe as S
+    from swalbe import serve
+    reqs = serve.poisson_requests(a.requests, rate=a.rate, seed=5)
+    free, total = S.device_memory()
+    cap = int(min(free * 0.8, 200 << 30))
+    be = serve.BatchBackend(cap, 4096, graph_steps=a.graph_steps)
+    # warm up the kernels and the graph path on a tiny film
+    w = serve.coalescence_request(-1, 128, 64, 2 * a.graph_steps + 3, np.random.default_rng(0), 1)
+    serve.Server(be, tick=a.tick).serve([w], realtime=False)
+    srv = serve.Server(be, tick=a.tick)
+    done = srv.serve(reqs)
+    be.close()
+    st = serve.latency_stats(done)
+    lups = sum(r.nodes * r.nsteps for r in reqs)
+    out = {"metric": "solves/s of continuous-batched FP32 coalescence solves (Poisson arrivals)",
+           "value": round(st["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+           "steps": a.requests, "warmup": 1, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded Poisson arrivals, seeded two-cap films)",
+           "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
+           "makespan_s": round(st["makespan_s"], 3), "batched_glups": round(st["glups"], 3),
+           "max_resident": srv.max_resident, "peak_arena_bytes": srv.peak_bytes}
+    if not a.no_alone:
+        t = time.perf_counter()
+        for r in reqs:
+            serve.solve_alone(r, "fused")
+        alone = time.perf_counter() - t
+        out["one_at_a_time"] = {"solves_per_s": round(len(reqs) / alone, 4),
+                                "seconds": round(alone, 3), "glups": round(lups / alone / 1e9, 3)}
+        out["speedup_vs_one_at_a_time"] = round(st["solves_per_s"] * alone / len(reqs), 3)
+    pick = np.random.default_rng(17).choice(len(done), size=max(1, round(0.05 * len(done))),
+                                            replace=False)
+    sample = [done[i] for i in sorted(pick)]
+    errs = []
+    for r in sample:
+        ref = serve.solve_alone(r, "fused", T=np.float64)["height"]
+        errs.append(float(np.linalg.norm(r.result["height"] - ref) / np.linalg.norm(ref)))
+    out["relL2_h_vs_fp64"] = {"sample": [f"{r.Lx}x{r.Ly} T={r.nsteps}" for r in sample],
+                              "max": max(errs)}
+    out["config"] = {"workload": f"serve_poisson: {a.requests} requests, lambda={a.rate}/s, "
+                                 "Lx in [128,2048] step 64, Ly in {Lx/2, Lx}, "
+                                 "T in {1e3,1e4,5e4}, two tangent caps, tau=1",
+                     "requests": a.requests, "lattice_updates": lups, "tick": a.tick,
+                     "graph_steps": a.graph_steps, "arena_bytes": cap,
+                     "parallelism": "single GPU"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="rupture4096_f32", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rupture4096_f32",
+                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS))
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="batch workloads: steps per captured HIP graph (0 = plain launches)")
+    ap.add_argument("--requests", type=int, default=48, help="serve_poisson: number of requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve_poisson: arrivals per second")
+    ap.add_argument("--tick", type=int, default=500, help="serve_poisson: steps per scheduler tick")
+    ap.add_argument("--no-alone", action="store_true",
+                    help="serve_poisson: skip the one-at-a-time baseline")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--nontemporal", type=int, default=0)
@@ -80,6 +189,11 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the films are independent or a single slab; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "coalesce256_batch":
+        return bench_batch(a)
+    if a.workload == "serve_poisson":
+        return bench_serve(a)
 
     import ctypes as C
     import numpy as np

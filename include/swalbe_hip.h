@@ -91,6 +91,10 @@ int swalbe_set_reference_height(swalbe_state *st, double href, double g);
  * with N = Lx*Ly (x9 for distributions).  Synchronous. */
 int swalbe_upload(swalbe_state *st, int32_t field, const void *host, size_t bytes);
 int swalbe_download(swalbe_state *st, int32_t field, void *host, size_t bytes);
+/* device-side copy of one field into another of the same kind (both
+ * distributions, both heights or both plain planes): the broadcast
+ * assignments of the drivers, e.g. `state.ftemp .= state.feq` */
+int swalbe_copy(swalbe_state *st, int32_t dst_field, int32_t src_field);
 /* state.<field> .= value (src/simulate.jl:241, test/collide.jl:19-22). */
 int swalbe_fill(swalbe_state *st, int32_t field, double value);
 int swalbe_synchronize(swalbe_state *st);
@@ -140,10 +144,9 @@ typedef struct swalbe_step_opts {
     double *dh_series;     /* host [nsteps]: max(h) - min(h) before each step (:56), or NULL */
     int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) (measures.jl:13-17), or NULL */
     double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
-    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8, 16 or 32 */
+    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8 or 16 */
     int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = collide at the source node,
-                              2 = collide at the destination node, 3 / 4 = the same as
-                              1 / 2 on a persistent, prefetching grid */
+                              2 = collide at the destination node */
     int32_t nontemporal;   /* FP32 kernel: streaming (non-temporal) stores of the outputs */
 } swalbe_step_opts;
 int swalbe_time_loop(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
@@ -162,6 +165,66 @@ int swalbe_reduce(swalbe_state *st, int32_t field, double thresh, double out[4])
  * (dist 0) or base + amp * N(0,1) (dist 1), Philox keyed (seed, node);
  * the counterpart of randinterface! (src/initialvalues.jl:23-33). */
 int swalbe_init_random(swalbe_state *st, int32_t dist, uint64_t seed, double base, double amp);
+
+/* ---- batches of independent FP32 films (no reference counterpart: the
+ * reference steps one film per process; these run many films of different
+ * sizes and parameters in one launch per time step).  A film is admitted
+ * with its parameters, an initial height and a step budget; it advances with
+ * every swalbe_batch_step until the budget is spent and keeps its state
+ * until removed.  Each film's arithmetic is the single-film FP32 kernel's,
+ * so a film's result is the same alone, in any mix, and via swalbe_time_loop.
+ * Initial height kinds follow src/initialvalues.jl (flat, randinterface!,
+ * singledroplet, two_droplets); τ = 1 films keep one distribution buffer.
+ * add / remove / step only enqueue work on the batch's stream (admissions
+ * never wait for the GPU); download waits for the film's last step only and
+ * copies on a second stream, so it overlaps later steps.  One batch is used
+ * by one host thread at a time. */
+#define SWALBE_INIT_FLAT 0     /* h = base */
+#define SWALBE_INIT_NORMAL 1   /* h = base + amp N(0,1) (Philox, seed) */
+#define SWALBE_INIT_CAP 2      /* spherical cap r1, θ1 at (c1x, c1y) on a precursor */
+#define SWALBE_INIT_TWO_CAPS 3 /* cap 1 + cap 2 - precursor */
+typedef struct swalbe_film_init {
+    int32_t kind;
+    int32_t thermal; /* subtract thermal noise from F (kbt from the params) */
+    int32_t body;    /* body force α h fac (inclination!) */
+    int32_t clamp;   /* caps: 1 = raise every h below the precursor to it; 0 = the
+                        reference's rule (only negative h becomes the precursor),
+                        which leaves h ≪ precursor at cap rims and necks */
+    uint64_t seed;   /* noise and initial-height stream */
+    double base, amp, precursor, href;
+    double r1, theta1, c1x, c1y; /* 1-based centres, θ in units of π */
+    double r2, theta2, c2x, c2y;
+    double body_ax, body_ay, body_fac;
+} swalbe_film_init;
+typedef struct swalbe_batch swalbe_batch;
+int swalbe_batch_create(int32_t dtype, size_t capacity_bytes, int32_t max_films, swalbe_batch **out);
+int swalbe_batch_destroy(swalbe_batch *b);
+/* device bytes one film takes in a batch arena */
+int swalbe_batch_film_bytes(int32_t Lx, int32_t Ly, double tau, size_t *bytes);
+int swalbe_batch_add(swalbe_batch *b, const swalbe_params *p, int32_t Lx, int32_t Ly,
+                     const swalbe_film_init *init, int64_t nsteps, int32_t *slot);
+int swalbe_batch_remove(swalbe_batch *b, int32_t slot);
+/* advance every resident film by nsteps (films stop at their budget) */
+int swalbe_batch_step(swalbe_batch *b, int64_t nsteps);
+int swalbe_batch_step_timed(swalbe_batch *b, int64_t nsteps, double *ms);
+/* replay runs of steps_per_graph steps as one captured HIP graph (0 = off) */
+int swalbe_batch_set_graph(swalbe_batch *b, int32_t steps_per_graph);
+int swalbe_batch_set_nontemporal(swalbe_batch *b, int32_t on);
+/* block until at most `steps_behind` of the enqueued batch steps are still
+ * pending on the device (bounds how far the host runs ahead) */
+int swalbe_batch_wait(swalbe_batch *b, int64_t steps_behind);
+int swalbe_batch_sync(swalbe_batch *b);
+int swalbe_batch_film_status(swalbe_batch *b, int32_t slot, int64_t *steps_done,
+                             int32_t *finished);
+/* field: SWALBE_HEIGHT, VELX, VELY, PRESSURE or FOUT; bytes as swalbe_download */
+int swalbe_batch_download(swalbe_batch *b, int32_t slot, int32_t field, void *host, size_t bytes);
+int swalbe_batch_stats(swalbe_batch *b, int32_t *nfilms, int64_t *nodes, size_t *bytes_used,
+                       size_t *bytes_free);
+/* initial height of a single-film state from the same spec as a batch film
+ * (singledroplet / two_droplets / randinterface!, src/initialvalues.jl) */
+int swalbe_init_height(swalbe_state *st, const swalbe_film_init *init);
+/* free and total device memory (admission by HBM footprint) */
+int swalbe_device_memory(size_t *free_bytes, size_t *total_bytes);
 
 /* ---- raw-array twins on caller-owned device arrays (plain values of the
  * given dtype, Julia layout), enqueued on `stream` (NULL = default stream).

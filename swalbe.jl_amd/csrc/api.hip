@@ -343,6 +343,28 @@ int swalbe_upload(swalbe_state* st, int32_t field, const void* host, size_t byte
   return copy_h2d(st, field, dst, tmp.data());
 }
 
+int swalbe_copy(swalbe_state* st, int32_t dst_field, int32_t src_field) {
+  SW_CHECK_STATE(st);
+  if (dst_field < 0 || dst_field >= SWALBE_NFIELDS || src_field < 0 ||
+      src_field >= SWALBE_NFIELDS || dst_field == SWALBE_THETA || src_field == SWALBE_THETA ||
+      is_dist(dst_field) != is_dist(src_field) || is_height(dst_field) != is_height(src_field)) {
+    set_error("swalbe_copy: fields must be both distributions or both the same kind of plane");
+    return SWALBE_EINVAL;
+  }
+  if (dst_field == src_field) return SWALBE_OK;
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  if (dst_field == SWALBE_FOUT || dst_field == SWALBE_FTEMP) unalias_for_write(F, dst_field);
+  int64_t c;
+  void* dst = field_ptr(st, dst_field, &c);
+  const void* src = field_ptr(st, src_field, &c);
+  if (dst == src) return SWALBE_OK;
+  const int64_t rows = is_dist(dst_field) ? kQ : 1;
+  SW_HIP(hipMemcpy2DAsync(dst, F.fstride * F.esz, src, F.fstride * F.esz, F.N * F.esz, rows,
+                          hipMemcpyDeviceToDevice, st->stream));
+  return SWALBE_OK;
+}
+
 int swalbe_download(swalbe_state* st, int32_t field, void* host, size_t bytes) {
   SW_CHECK_STATE(st);
   if (field < 0 || field >= SWALBE_NFIELDS || !host) {
@@ -599,6 +621,30 @@ int swalbe_reduce(swalbe_state* st, int32_t field, double thresh, double out[4])
   return SWALBE_OK;
 }
 
+int swalbe_init_height(swalbe_state* st, const swalbe_film_init* init) {
+  SW_CHECK_STATE(st);
+  if (!init || init->kind < SWALBE_INIT_FLAT || init->kind > SWALBE_INIT_TWO_CAPS) {
+    set_error("swalbe_init_height: bad init spec");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  // the same offset as uploads: the FP32 reference height the kernels add
+  const double off = enc_offset(F, SWALBE_HEIGHT, 0);
+  SW_BOTH(st, launch_film_init<T>((T*)F.height(), F.Lx, F.Ly, *init, off, st->stream));
+  SW_HIP(hipGetLastError());
+  return SWALBE_OK;
+}
+
+int swalbe_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  size_t f = 0, t = 0;
+  hipError_t e = hipMemGetInfo(&f, &t);
+  if (e != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return SWALBE_OK;
+}
+
 int swalbe_init_random(swalbe_state* st, int32_t dist, uint64_t seed, double base, double amp) {
   SW_CHECK_STATE(st);
   if (dist != 0 && dist != 1) {
@@ -607,7 +653,7 @@ int swalbe_init_random(swalbe_state* st, int32_t dist, uint64_t seed, double bas
   }
   if (int rc = set_device(st)) return rc;
   Film& F = st->film;
-  const double off = F.dtype == SWALBE_F32 ? F.href : 0.0;
+  const double off = enc_offset(F, SWALBE_HEIGHT, 0);
   SW_BOTH(st, launch_init_random<T>((T*)F.height(), dist, seed, base, amp, off, F.Lx, F.Ly,
                                     st->stream));
   SW_HIP(hipGetLastError());
@@ -617,33 +663,7 @@ int swalbe_init_random(swalbe_state* st, int32_t dist, uint64_t seed, double bas
 }  // extern "C"
 
 // ---------------------------------------------------------------- fused time loop
-namespace {
-
-__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
-__global__ void k_init_keys(unsigned long long* p, int64_t nsteps) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nsteps;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    p[2 * i] = 0ull;
-    p[2 * i + 1] = ~0ull;
-  }
-}
-
-struct LoopBuffers {
-  unsigned long long* keys = nullptr;  // 2 per step
-  unsigned long long* wet = nullptr;   // 1 per step
-  double* mass = nullptr;              // 4 per tdump
-  ~LoopBuffers() {
-    if (keys) hipFree(keys);
-    if (wet) hipFree(wet);
-    if (mass) hipFree(mass);
-  }
-};
-
+namespace swalbe {
 StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, int powmode) {
   StepParams sp{};
   sp.gamma = p->gamma;
@@ -674,6 +694,35 @@ StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, i
   sp.wet_thresh = o ? o->wet_thresh : 0.0;
   return sp;
 }
+
+}  // namespace swalbe
+
+namespace {
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_init_keys(unsigned long long* p, int64_t nsteps) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nsteps;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    p[2 * i] = 0ull;
+    p[2 * i + 1] = ~0ull;
+  }
+}
+
+struct LoopBuffers {
+  unsigned long long* keys = nullptr;  // 2 per step
+  unsigned long long* wet = nullptr;   // 1 per step
+  double* mass = nullptr;              // 4 per tdump
+  ~LoopBuffers() {
+    if (keys) hipFree(keys);
+    if (wet) hipFree(wet);
+    if (mass) hipFree(mass);
+  }
+};
 
 template <typename T>
 void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64_t t, int tile_y,
@@ -746,12 +795,12 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
     return SWALBE_EINVAL;
   }
   const int tile_y = o->tile_y > 0 ? o->tile_y : 16;
-  if (tile_y != 8 && tile_y != 16 && tile_y != 32) {
-    set_error("swalbe_time_loop: tile_y must be 8, 16 or 32");
+  if (tile_y != 8 && tile_y != 16) {
+    set_error("swalbe_time_loop: tile_y must be 8 or 16");
     return SWALBE_EINVAL;
   }
-  if (o->variant < 0 || o->variant > 4) {
-    set_error("swalbe_time_loop: variant must be 0..4");
+  if (o->variant < 0 || o->variant > 2) {
+    set_error("swalbe_time_loop: variant must be 0, 1 or 2");
     return SWALBE_EINVAL;
   }
   const StepParams sp = make_step_params(p, o, 0);
@@ -781,7 +830,7 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
   for (int64_t it = 1; it <= nsteps; ++it) {
     const int64_t t = o->t0 + it;
     if (want_mass && t % p->tdump == 0) {
-      if (int rc = reduce_enqueue(st, F.height(), F.dtype == SWALBE_F32 ? F.href : 0.0, 0.0,
+      if (int rc = reduce_enqueue(st, F.height(), enc_offset(F, SWALBE_HEIGHT, 0), 0.0,
                                   lb.mass + 4 * km))
         return rc;
       ++km;

@@ -1,0 +1,48 @@
+// swalbe_batch.h -- FP32 step constants and the device descriptors of a
+// batch of independent films (shared by step_f32.hip and batch.hip).
+#pragma once
+
+#include "swalbe_internal.h"
+
+namespace swalbe {
+
+// FP32 constants of one film's step.
+struct F32K {
+  float gamma, kap, hmin, hcrit, c6mu, c6d, c3dd, kb26, omeg, itau, c56g, g0, href;
+  float bfx, bfy;  // body force α·fac
+  int n, m, powmode;
+};
+
+F32K make_f32k(const StepParams& sp, const Enc& enc);
+int film_is_small(int Lx, int Ly, int tile_y);
+
+// Rows per tile of the batched kernel.
+constexpr int kBatchTileY = 16;
+
+// One resident film of a batch (device memory, read with scalar loads).
+struct BatchFilm {
+  float* h[2];
+  float* ux[2];
+  float* uy[2];
+  float* p;
+  float* f[2];  // f[1] only when τ ≠ 1
+  int Lx, Ly, ntx, ntiles, small, pairs, read_ft, thermal, body, pad_;
+  int64_t fstride;
+  uint64_t seed;
+  int64_t t0;                // step number before admission
+  unsigned long long admit;  // launch counter value at admission
+  unsigned long long end;    // launch counter value at which the film stops
+  F32K K;
+};
+
+struct BatchHdr {
+  int nfilms;
+  int total_tiles;
+};
+
+// One batched step (all resident films) followed by the counter increment.
+void launch_batch_f32(const BatchFilm* films, const int* tstart, const BatchHdr* hdr,
+                      unsigned long long* counter, int grid, int nts, hipStream_t s);
+int batch_blocks_per_cu();
+
+}  // namespace swalbe

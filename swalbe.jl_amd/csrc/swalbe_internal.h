@@ -99,6 +99,10 @@ void launch_fill(T* p, double v, int64_t N, hipStream_t s);
 template <typename T>
 void launch_init_random(T* h, int dist, uint64_t seed, double base, double amp, double href,
                         int Lx, int Ly, hipStream_t s);
+// Initial height of a film from a swalbe_film_init spec (stored as h - href).
+template <typename T>
+void launch_film_init(T* h, int Lx, int Ly, const swalbe_film_init& in, double href,
+                      hipStream_t s);
 // κ(θ) per node from a θ field (same arithmetic as the scalar κ).
 template <typename T>
 void launch_kappa_field(T* kap, const T* theta, int n, int m, double hmin, int64_t N,
@@ -120,6 +124,8 @@ struct StepParams {
   int materialize;
   double wet_thresh;
 };
+
+StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, int powmode);
 
 struct StepSeries {
   unsigned long long* dh_keys;  // [2] ordered keys of max / min h, or null

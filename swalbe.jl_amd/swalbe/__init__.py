@@ -17,6 +17,8 @@ from .core import (BGKandStream, DeviceArray, HipState, Sys, SysConst, Taumucs, 
                    wetted)
 from .initialvalues import (normals, randinterface, random_film, rivulet, sinewave, singledroplet,
                             torus, two_droplets_2d, uniforms)
+from .batch import (INIT_CAP, INIT_FLAT, INIT_NORMAL, INIT_TWO_CAPS, Batch, FilmInit, device_memory,
+                    film_bytes, film_init)
 from .simulate import (run_dropletforced, run_dropletpatterned, run_dropletrelax, run_flat,
                        run_random, run_rayleightaylor)
 

@@ -24,7 +24,12 @@ DIST_FIELDS = ("fout", "ftemp", "feq")
 
 
 class SwalbeError(RuntimeError):
-    """A HIP runtime, allocation or argument error reported by the library."""
+    """A HIP runtime, allocation or argument error reported by the library;
+    `status` is the library's status code (ENOMEM, EINVAL, EHIP)."""
+
+    def __init__(self, msg: str, status: int = 0):
+        super().__init__(msg)
+        self.status = status
 
 
 class DomainError(ValueError):
@@ -58,6 +63,18 @@ _D = C.c_double
 _I = C.c_int32
 
 
+class FilmInit(C.Structure):
+    """swalbe_film_init: initial height spec of a film (flat, noisy, one or two caps)."""
+    _fields_ = [
+        ("kind", C.c_int32), ("thermal", C.c_int32), ("body", C.c_int32), ("clamp", C.c_int32),
+        ("seed", C.c_uint64), ("base", C.c_double), ("amp", C.c_double),
+        ("precursor", C.c_double), ("href", C.c_double),
+        ("r1", C.c_double), ("theta1", C.c_double), ("c1x", C.c_double), ("c1y", C.c_double),
+        ("r2", C.c_double), ("theta2", C.c_double), ("c2x", C.c_double), ("c2y", C.c_double),
+        ("body_ax", C.c_double), ("body_ay", C.c_double), ("body_fac", C.c_double),
+    ]
+
+
 def _sig(L, name, args):
     fn = getattr(L, name)
     fn.restype = C.c_int
@@ -84,6 +101,7 @@ def lib():
     _sig(L, "swalbe_upload", [_V, _I, _V, C.c_size_t])
     _sig(L, "swalbe_download", [_V, _I, _V, C.c_size_t])
     _sig(L, "swalbe_fill", [_V, _I, _D])
+    _sig(L, "swalbe_copy", [_V, _I, _I])
     _sig(L, "swalbe_synchronize", [_V])
     _sig(L, "swalbe_equilibrium", [_V, _D])
     _sig(L, "swalbe_bgk_and_stream", [_V, _D])
@@ -99,6 +117,24 @@ def lib():
          [_V, C.POINTER(Params), C.c_int64, C.POINTER(StepOpts), C.POINTER(_D)])
     _sig(L, "swalbe_reduce", [_V, _I, _D, C.POINTER(_D)])
     _sig(L, "swalbe_init_random", [_V, _I, C.c_uint64, _D, _D])
+    _sig(L, "swalbe_init_height", [_V, C.POINTER(FilmInit)])
+    _sig(L, "swalbe_device_memory", [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)])
+    _sig(L, "swalbe_batch_create", [_I, C.c_size_t, _I, C.POINTER(_V)])
+    _sig(L, "swalbe_batch_destroy", [_V])
+    _sig(L, "swalbe_batch_film_bytes", [_I, _I, _D, C.POINTER(C.c_size_t)])
+    _sig(L, "swalbe_batch_add", [_V, C.POINTER(Params), _I, _I, C.POINTER(FilmInit), C.c_int64,
+                                 C.POINTER(_I)])
+    _sig(L, "swalbe_batch_remove", [_V, _I])
+    _sig(L, "swalbe_batch_step", [_V, C.c_int64])
+    _sig(L, "swalbe_batch_step_timed", [_V, C.c_int64, C.POINTER(_D)])
+    _sig(L, "swalbe_batch_set_graph", [_V, _I])
+    _sig(L, "swalbe_batch_set_nontemporal", [_V, _I])
+    _sig(L, "swalbe_batch_sync", [_V])
+    _sig(L, "swalbe_batch_wait", [_V, C.c_int64])
+    _sig(L, "swalbe_batch_film_status", [_V, _I, C.POINTER(C.c_int64), C.POINTER(_I)])
+    _sig(L, "swalbe_batch_download", [_V, _I, _I, _V, C.c_size_t])
+    _sig(L, "swalbe_batch_stats", [_V, C.POINTER(_I), C.POINTER(C.c_int64), C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t)])
     _sig(L, "swalbe_equilibrium_ptr", [_I, _V, _V, _V, _V, _V, _D, _I, _I, _V])
     _sig(L, "swalbe_bgk_and_stream_ptr", [_I, _V, _V, _V, _V, _V, _D, _I, _I, _V])
     _sig(L, "swalbe_moments_ptr", [_I, _V, _V, _V, _V, _I, _I, _V])
@@ -127,7 +163,7 @@ def check(rc: int, what: str = "") -> None:
     msg = lib().swalbe_last_error().decode(errors="replace")
     if rc == EDOMAIN:
         raise DomainError(msg)
-    raise SwalbeError(f"{what}: {msg} (status {rc})")
+    raise SwalbeError(f"{what}: {msg} (status {rc})", rc)
 
 
 def device_count() -> int:

@@ -161,6 +161,15 @@ class HipState:
         """FP32 storage offset (deviation encoding); no effect for FP64."""
         check(lib().swalbe_set_reference_height(self._h, float(href), float(g)), "set_reference_height")
 
+    def copy(self, dst: str, src: str) -> None:
+        """Device-side `dst .= src` between fields of the same kind."""
+        check(lib().swalbe_copy(self._h, self._names[dst], self._names[src]), "copy")
+
+    def init_height(self, init) -> None:
+        """Initial height from a FilmInit spec (the same kernel a batch film
+        is initialised with)."""
+        check(lib().swalbe_init_height(self._h, C.byref(init)), "init_height")
+
     def reduce(self, name: str = "height", thresh: float = 0.0):
         out = (C.c_double * 4)()
         check(lib().swalbe_reduce(self._h, self._names[name], float(thresh), out), "reduce")

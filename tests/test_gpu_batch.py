@@ -1,0 +1,225 @@
+"""Batched FP32 films on the GPU: each film of a batch gives the same bits as
+the same film stepped alone (single-film fused loop), whatever the mix, the
+admission time, the graph capture and the arena placement; films agree
+with the FP64 oracle to the FP32 tolerance; the serving loop returns the
+alone results."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import swalbe
+    if swalbe.device_count() < 1:
+        pytest.skip("no GPU")
+    return swalbe
+
+
+def relL2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def caps_host(Lx, Ly, fi):
+    """Host restatement of the cap initial height (src/initialvalues.jl:203-224,
+    two caps as cap1 + cap2 - precursor, :277-323), 1-based centres."""
+    i = np.arange(1, Lx + 1)[:, None]
+    j = np.arange(1, Ly + 1)[None, :]
+
+    def cap(R, th, cx, cy):
+        r = np.sqrt((i - cx) ** 2 + (j - cy) ** 2)
+        s = np.minimum(r / R, 1.0)
+        h = np.where(r <= R, (np.sqrt(1 - s * s) - math.cos(math.pi * th)) * R, fi.precursor)
+        return np.where(h < 0, fi.precursor, h)
+
+    h = cap(fi.r1, fi.theta1, fi.c1x, fi.c1y)
+    if fi.kind == 3:
+        h = h + cap(fi.r2, fi.theta2, fi.c2x, fi.c2y) - fi.precursor
+        h = np.where(h < 0, fi.precursor, h)
+    if fi.clamp:
+        h = np.maximum(h, fi.precursor)
+    return h
+
+
+def film_specs(S):
+    """(Lx, Ly, Taumucs, FilmInit, nsteps): a mix of sizes and features."""
+    T = S.Taumucs
+    fi = S.film_init
+    specs = [
+        (256, 128, T(Tmax=1, γ=0.01, θ=1 / 9), fi(S.INIT_TWO_CAPS, precursor=0.1, href=0.1,
+                                                 cap1=(40.0, 1 / 9, 114.8, 64.5),
+                                                 cap2=(40.0, 1 / 9, 142.2, 64.5),
+                                                 clamp=True), 300),
+        (200, 100, T(Tmax=1, kbt=1e-6, γ=0.01), fi(S.INIT_NORMAL, thermal=True, seed=11, base=1.0,
+                                                   amp=1e-3, href=1.0), 300),
+        (199, 61, T(Tmax=1, kbt=1e-5, γ=0.02), fi(S.INIT_NORMAL, thermal=True, seed=12, base=1.0,
+                                                  amp=1e-3, href=1.0), 200),
+        (40, 20, T(Tmax=1, kbt=1e-6), fi(S.INIT_NORMAL, thermal=True, seed=13, base=1.0,
+                                         amp=1e-3, href=1.0), 300),
+        (192, 96, T(Tmax=1, τ=0.8, γ=0.01, θ=1 / 6), fi(S.INIT_CAP, precursor=0.1, href=0.1,
+                                                       cap1=(30.0, 1 / 6, 96.5, 48.5)), 250),
+        (256, 96, T(Tmax=1, γ=0.01), fi(S.INIT_NORMAL, seed=14, base=1.0, amp=1e-3, href=1.0,
+                                        body=(2e-5, -1e-5, 1.0)), 300),
+        (64, 64, T(Tmax=1, kbt=1e-6, γ=0.005), fi(S.INIT_CAP, thermal=True, seed=15,
+                                                  precursor=0.1, href=0.1, clamp=True,
+                                                  cap1=(20.0, 1 / 12, 32.5, 32.5)), 100),
+        (130, 22, T(Tmax=1, γ=0.01), fi(S.INIT_FLAT, base=0.5, href=0.5), 50),
+    ]
+    return specs
+
+
+FIELDS = ("height", "velx", "vely", "pressure", "fout")
+
+
+def alone(S, Lx, Ly, prm, init, nsteps):
+    from swalbe import serve
+    r = serve.Request(0, Lx, Ly, prm, init, nsteps, fields=FIELDS)
+    return serve.solve_alone(r, "fused")
+
+
+def assert_same(got, ref, what):
+    for name in FIELDS:
+        a, b = got[name], ref[name]
+        assert a.shape == b.shape, (what, name)
+        bad = np.flatnonzero(a.ravel(order="F") != b.ravel(order="F"))
+        assert bad.size == 0, f"{what}: {name} differs at {bad.size} entries, first {bad[:5]}"
+
+
+def test_initial_height_matches_host_formula(S):
+    for Lx, Ly, prm, init, _ in film_specs(S):
+        if init.kind not in (S.INIT_CAP, S.INIT_TWO_CAPS):
+            continue
+        b = S.Batch(64 << 20, 8)
+        slot = b.add(prm, Lx, Ly, init, 0)
+        h = b.get(slot, "height")
+        b.close()
+        ref = caps_host(Lx, Ly, init)
+        assert np.max(np.abs(h - ref)) <= 4e-6 * max(1.0, ref.max()), (Lx, Ly)
+
+
+def test_batch_matches_each_film_alone_bitwise(S):
+    specs = film_specs(S)
+    b = S.Batch(1 << 30, 64)
+    slots = [b.add(prm, Lx, Ly, init, n) for Lx, Ly, prm, init, n in specs]
+    b.step(max(s[4] for s in specs) + 7)  # films stop at their own budgets
+    for slot, (Lx, Ly, prm, init, n) in zip(slots, specs):
+        print(f"[batch] film {Lx}x{Ly} tau {prm.τ} steps {n}", flush=True)
+        assert b.status(slot) == (n, True)
+        got = {name: b.get(slot, name) for name in FIELDS}
+        assert_same(got, alone(S, Lx, Ly, prm, init, n), f"{Lx}x{Ly}")
+    b.close()
+
+
+def test_staggered_admission_and_slot_reuse(S):
+    """Films admitted after others have run, into memory freed by a retired
+    film, still give their alone bits; graph replays and single launches mix."""
+    specs = film_specs(S)
+    b = S.Batch(96 << 20, 16, graph_steps=32)
+    A = b.add(*_args(specs[0]))
+    b.step(70)
+    B = b.add(*_args(specs[1]))
+    C_ = b.add(*_args(specs[3]))
+    b.step(100)
+    D = b.add(*_args(specs[4]))
+    b.step(40)
+    assert b.status(C_) == (140, False)
+    b.step(200)
+    # A: 300 steps, done at launch 300; B, C: 300 steps from launch 70
+    for slot, sp in ((A, specs[0]), (B, specs[1]), (C_, specs[3]), (D, specs[4])):
+        Lx, Ly, prm, init, n = sp
+        assert b.status(slot)[0] == min(n, {A: 410, B: 340, C_: 340, D: 240}[slot])
+    got_a = {name: b.get(A, name) for name in FIELDS}
+    assert_same(got_a, alone(S, *specs[0]), "A")
+    b.remove(A)
+    E = b.add(*_args(specs[2]))  # reuses A's memory while B, C, D keep running
+    b.step(300)
+    for slot, sp, what in ((B, specs[1], "B"), (C_, specs[3], "C"), (D, specs[4], "D"),
+                           (E, specs[2], "E")):
+        got = {name: b.get(slot, name) for name in FIELDS}
+        assert_same(got, alone(S, *sp), what)
+    st = b.stats()
+    assert st["films"] == 4
+    b.close()
+
+
+def _args(sp):
+    Lx, Ly, prm, init, n = sp
+    return prm, Lx, Ly, init, n
+
+
+def test_graph_and_plain_launches_agree(S):
+    specs = film_specs(S)[:4]
+    outs = []
+    for g in (0, 16, 64):
+        b = S.Batch(256 << 20, 8, graph_steps=g)
+        slots = [b.add(*_args(sp)) for sp in specs]
+        for n in (5, 64, 100, 131):
+            b.step(n)
+        outs.append([b.get(s, "height") for s in slots])
+        b.close()
+    for o in outs[1:]:
+        for a, ref in zip(o, outs[0]):
+            assert np.array_equal(a, ref)
+
+
+def test_batch_film_vs_fp64_oracle(S):
+    """FP32 batch film against the FP64 oracle from the same initial height:
+    rel-L2 of h <= 1e-5 after 400 steps; mass to 1e-6."""
+    Lx, Ly, prm, init, _ = film_specs(S)[0]
+    b = S.Batch(64 << 20, 4)
+    slot = b.add(prm, Lx, Ly, init, 400)
+    h0 = b.get(slot, "height")
+    b.step(400)
+    h = b.get(slot, "height")
+    b.close()
+    ost = O.State(Lx, Ly)
+    ost.height[...] = h0
+    ost.feq[...], _ = O.equilibrium(ost.height, ost.velx, ost.vely, 0.0)
+    p = O.params(gamma=prm.γ, theta=prm.θ, tdump=10 ** 9)
+    O.time_loop(ost, p, 400)
+    e = relL2(h, ost.height)
+    print(f"[batch] fp32 vs fp64 oracle rel-L2 h = {e:.3e}", flush=True)
+    assert e <= 1e-5
+    assert abs(h.sum() - ost.height.sum()) <= 1e-6 * ost.height.sum()
+
+
+def test_batch_errors(S):
+    from swalbe._lib import SwalbeError
+    b = S.Batch(1 << 20, 2)
+    prm = S.Taumucs(Tmax=1)
+    fi = S.film_init(S.INIT_FLAT, base=1.0, href=1.0)
+    with pytest.raises(SwalbeError, match="no room"):
+        b.add(prm, 1024, 1024, fi, 10)
+    s0 = b.add(prm, 32, 16, fi, 10)
+    b.add(prm, 32, 16, fi, 10)
+    with pytest.raises(SwalbeError, match="slot"):
+        b.add(prm, 32, 16, fi, 10)
+    b.remove(s0)
+    with pytest.raises(SwalbeError):
+        b.remove(s0)
+    b.step(3)
+    assert b.stats()["films"] == 1
+    b.close()
+
+
+def test_serving_results_equal_alone(S):
+    """Continuous batching with staggered arrivals: every served result is
+    bit-identical to the same request solved alone."""
+    from swalbe import serve
+    reqs = serve.poisson_requests(12, rate=2000.0, seed=9, lx_range=(128, 320),
+                                  steps=(100, 300, 600))
+    be = serve.BatchBackend(1 << 30, 64, graph_steps=50)
+    srv = serve.Server(be, tick=120)
+    done = srv.serve(reqs)
+    be.close()
+    assert len(done) == len(reqs)
+    assert srv.max_resident > 1
+    for r in done:
+        ref = serve.solve_alone(r, "fused")
+        assert np.array_equal(r.result["height"], ref["height"]), r.rid
