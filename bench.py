@@ -32,8 +32,8 @@ WORKLOADS = {
     "flat4096_f32_nonoise": (4096, 4096, "f32", False),
     "droplet1024_f32": (1024, 1024, "f32", False),
 }
-# many-film workloads (FP32 batched kernel)
-BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson")
+# many-film workloads (FP32 batched kernel) and the copy calibration
+BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson", "copy4096_f32")
 
 
 def bytes_per_lup(esz: int, tau_is_one: bool) -> int:
@@ -108,6 +108,33 @@ def bench_batch(a):
                      "traffic": None, "bytes_per_lup": bpl},
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_copy(a):
+    """Device-to-device copy of the nine distribution planes of a 4096^2 FP32
+    state (feq -> ftemp, 604 MB each way): the achievable HBM bandwidth of a
+    plain copy, and the known-size kernel that calibrates the byte counters."""
+    import swalbe as S
+    sysc = S.SysConst(4096, 4096, S.Taumucs(Tmax=1))
+    import numpy as np
+    st = S.Sys(sysc, "GPU", T=np.float32)
+    for _ in range(max(1, a.warmup)):
+        st.copy("ftemp", "feq")
+    st.synchronize()
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        st.copy("ftemp", "feq")
+    st.synchronize()
+    dt = time.perf_counter() - t
+    nbytes = 9 * 4096 * 4096 * 4
+    gbs = 2 * nbytes * a.steps / dt / 1e9
+    print(json.dumps({"metric": "device-to-device copy bandwidth (read + write)",
+                      "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": a.steps,
+                      "warmup": a.warmup, "ms_per_step": round(dt * 1e3 / a.steps, 5),
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "copy4096_f32: 9 planes of 4096^2 FP32",
+                                 "bytes_per_copy": nbytes}}), flush=True)
 
 
 def bench_serve(a):
@@ -194,6 +221,8 @@ def main():
         return bench_batch(a)
     if a.workload == "serve_poisson":
         return bench_serve(a)
+    if a.workload == "copy4096_f32":
+        return bench_copy(a)
 
     import ctypes as C
     import numpy as np

@@ -220,6 +220,25 @@ int swalbe_batch_film_status(swalbe_batch *b, int32_t slot, int64_t *steps_done,
 int swalbe_batch_download(swalbe_batch *b, int32_t slot, int32_t field, void *host, size_t bytes);
 int swalbe_batch_stats(swalbe_batch *b, int32_t *nfilms, int64_t *nodes, size_t *bytes_used,
                        size_t *bytes_free);
+/* ---- serving: continuous batching of solve requests over a batch (no
+ * reference counterpart; the reference runs sweeps as a serial host loop,
+ * scripts/Rivulet_stability.jl:141-190).  Requests are admitted in
+ * submission order while their film fits the arena and a slot is free.
+ * Each poll admits, enqueues one tick of steps (the tick cut to the smallest
+ * remaining budget) and hands back films that finished in an earlier tick;
+ * finished heights wait on the host until taken (peek, then take). */
+typedef struct swalbe_server swalbe_server;
+int swalbe_serve_create(size_t capacity_bytes, int32_t max_films, int32_t tick_steps,
+                        int32_t graph_steps, swalbe_server **out);
+int swalbe_serve_destroy(swalbe_server *s);
+int swalbe_serve_submit(swalbe_server *s, const swalbe_params *p, int32_t Lx, int32_t Ly,
+                        const swalbe_film_init *init, int64_t nsteps, int64_t *request_id);
+int swalbe_serve_poll(swalbe_server *s, int32_t *n_done, int32_t *n_resident, int32_t *n_queued);
+/* id and size of the oldest finished request */
+int swalbe_serve_peek(swalbe_server *s, int64_t *request_id, int32_t *Lx, int32_t *Ly);
+/* its final height, Lx*Ly doubles column-major; removes it */
+int swalbe_serve_take(swalbe_server *s, double *height, size_t bytes);
+
 /* initial height of a single-film state from the same spec as a batch film
  * (singledroplet / two_droplets / randinterface!, src/initialvalues.jl) */
 int swalbe_init_height(swalbe_state *st, const swalbe_film_init *init);

@@ -103,31 +103,49 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
   constexpr int UC = SM::UC, UR = SM::UR, UN = SM::UN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wv = tid >> 6;
+  // wave index made provably uniform: row offsets then live in scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int Lx = R.Lx, Ly = R.Ly;
   const int64_t FS = R.fstride;
   const int small = R.small;
   auto wx = [&](int a) { return small ? wrapm(a, Lx) : wrap1(a, Lx); };
   auto wy = [&](int a) { return small ? wrapm(a, Ly) : wrap1(a, Ly); };
 
-  // ---- A: halo rows of h (3) and u (1), one wavefront per row
+  // ---- A: h on the 3-halo and u on the 1-halo, flat over the block; all
+  // loads are issued before the first LDS store (memory-level parallelism)
   {
-    const int ca = wx(x0 - 3 + lane);
-    const int cb = wx(x0 + 61 + (lane & 7));  // columns 64..69 for lanes 0..5
-    for (int r = wv; r < HR; r += NW) {
-      const float* row = io.h_in + (int64_t)Lx * wy(y0 - 3 + r);
-      sm.h[r * HC + lane] = row[ca];
-      if (lane < 6) sm.h[r * HC + 64 + lane] = row[cb];
+    constexpr int HN = SM::HN;
+    constexpr int NHI = (HN + NT - 1) / NT, NUI = (UN + NT - 1) / NT;
+    float hreg[NHI], ureg[NUI], vreg[NUI];
+#pragma unroll
+    for (int i = 0; i < NHI; ++i) {
+      const int e = tid + i * NT;
+      if (e < HN) {
+        const int r = e / HC, c = e - r * HC;
+        hreg[i] = io.h_in[wx(x0 - 3 + c) + (int64_t)Lx * wy(y0 - 3 + r)];
+      }
     }
-    const int ua = wx(x0 - 1 + lane);
-    const int ub = wx(x0 + 63 + (lane & 1));
-    for (int r = wv; r < UR; r += NW) {
-      const int64_t ro = (int64_t)Lx * wy(y0 - 1 + r);
-      sm.u[r * UC + lane] = io.ux_in[ro + ua];
-      sm.v[r * UC + lane] = io.uy_in[ro + ua];
-      if (lane < 2) {
-        sm.u[r * UC + 64 + lane] = io.ux_in[ro + ub];
-        sm.v[r * UC + 64 + lane] = io.uy_in[ro + ub];
+#pragma unroll
+    for (int i = 0; i < NUI; ++i) {
+      const int e = tid + i * NT;
+      if (e < UN) {
+        const int r = e / UC, c = e - r * UC;
+        const int64_t g = wx(x0 - 1 + c) + (int64_t)Lx * wy(y0 - 1 + r);
+        ureg[i] = io.ux_in[g];
+        vreg[i] = io.uy_in[g];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NHI; ++i) {
+      const int e = tid + i * NT;
+      if (e < HN) sm.h[e] = hreg[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NUI; ++i) {
+      const int e = tid + i * NT;
+      if (e < UN) {
+        sm.u[e] = ureg[i];
+        sm.v[e] = vreg[i];
       }
     }
   }

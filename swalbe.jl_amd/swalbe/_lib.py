@@ -133,6 +133,13 @@ def lib():
     _sig(L, "swalbe_batch_wait", [_V, C.c_int64])
     _sig(L, "swalbe_batch_film_status", [_V, _I, C.POINTER(C.c_int64), C.POINTER(_I)])
     _sig(L, "swalbe_batch_download", [_V, _I, _I, _V, C.c_size_t])
+    _sig(L, "swalbe_serve_create", [C.c_size_t, _I, _I, _I, C.POINTER(_V)])
+    _sig(L, "swalbe_serve_destroy", [_V])
+    _sig(L, "swalbe_serve_submit", [_V, C.POINTER(Params), _I, _I, C.POINTER(FilmInit), C.c_int64,
+                                    C.POINTER(C.c_int64)])
+    _sig(L, "swalbe_serve_poll", [_V, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)])
+    _sig(L, "swalbe_serve_peek", [_V, C.POINTER(C.c_int64), C.POINTER(_I), C.POINTER(_I)])
+    _sig(L, "swalbe_serve_take", [_V, C.POINTER(_D), C.c_size_t])
     _sig(L, "swalbe_batch_stats", [_V, C.POINTER(_I), C.POINTER(C.c_int64), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)])
     _sig(L, "swalbe_equilibrium_ptr", [_I, _V, _V, _V, _V, _V, _D, _I, _I, _V])

@@ -280,6 +280,84 @@ class Server:
         return self.done
 
 
+class NativeServer:
+    """The same scheduler in the library (swalbe_serve_*): for callers that
+    drive serving through the C ABI alone."""
+
+    def __init__(self, capacity_bytes: int, max_films: int = 4096, tick: int = 500,
+                 graph_steps: int = 0):
+        import ctypes as C
+
+        from ._lib import check, lib
+        self._C, self._check, self._lib = C, check, lib
+        h = C.c_void_p()
+        check(lib().swalbe_serve_create(int(capacity_bytes), int(max_films), int(tick),
+                                        int(graph_steps), C.byref(h)), "swalbe_serve_create")
+        self._h = h
+        self.requests: Dict[int, Request] = {}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib().swalbe_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, r: Request) -> int:
+        C = self._C
+        rid = C.c_int64()
+        prm = r.param.cparams()
+        self._check(self._lib().swalbe_serve_submit(self._h, C.byref(prm), r.Lx, r.Ly,
+                                                    C.byref(r.init), int(r.nsteps), C.byref(rid)),
+                    "swalbe_serve_submit")
+        self.requests[rid.value] = r
+        return rid.value
+
+    def poll(self) -> List[Request]:
+        """One tick; returns the requests finished since the last poll."""
+        C = self._C
+        nd, nr, nq = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self._lib().swalbe_serve_poll(self._h, C.byref(nd), C.byref(nr), C.byref(nq)),
+                    "swalbe_serve_poll")
+        self.resident, self.queued = nr.value, nq.value
+        out = []
+        for _ in range(nd.value):
+            rid, Lx, Ly = C.c_int64(), C.c_int32(), C.c_int32()
+            self._check(self._lib().swalbe_serve_peek(self._h, C.byref(rid), C.byref(Lx),
+                                                      C.byref(Ly)), "swalbe_serve_peek")
+            h = np.empty((Lx.value, Ly.value), order="F")
+            self._check(self._lib().swalbe_serve_take(
+                self._h, h.ctypes.data_as(C.POINTER(C.c_double)), h.nbytes), "swalbe_serve_take")
+            r = self.requests.pop(rid.value)
+            r.result["height"] = h
+            out.append(r)
+        return out
+
+    def serve(self, requests: Sequence[Request], clock=time.perf_counter) -> List[Request]:
+        """Submit each request at its arrival time and poll until all are done."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        t0 = clock()
+        done: List[Request] = []
+        self.max_resident = 0
+        while pending or self.requests:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                r = pending.popleft()
+                self.submit(r)
+            if not self.requests:
+                time.sleep(min(max(pending[0].arrival - (clock() - t0), 0.0), 0.01))
+                continue
+            for r in self.poll():
+                r.t_done = clock() - t0
+                done.append(r)
+            self.max_resident = max(self.max_resident, self.resident)
+        return done
+
+
 def latency_stats(reqs: Sequence[Request]) -> dict:
     lat = np.array([r.latency for r in reqs])
     first = min(r.arrival for r in reqs)
@@ -335,6 +413,6 @@ def solve_alone(r: Request, mode: str = "fused", T=np.float32) -> Dict[str, np.n
     return out
 
 
-__all__ = ["Request", "Server", "BatchBackend", "coalescence_sweep", "poisson_requests",
+__all__ = ["Request", "Server", "NativeServer", "BatchBackend", "coalescence_sweep", "poisson_requests",
            "coalescence_request", "two_caps_film", "solve_alone", "latency_stats", "domain_seed",
            "splitmix64"]

@@ -223,3 +223,18 @@ def test_serving_results_equal_alone(S):
     for r in done:
         ref = serve.solve_alone(r, "fused")
         assert np.array_equal(r.result["height"], ref["height"]), r.rid
+
+
+def test_native_server_results_equal_alone(S):
+    """The library's own scheduler (swalbe_serve_*) gives the alone results."""
+    from swalbe import serve
+    reqs = serve.poisson_requests(10, rate=2000.0, seed=21, lx_range=(128, 256),
+                                  steps=(50, 200, 400))
+    srv = serve.NativeServer(1 << 30, 64, tick=90, graph_steps=32)
+    done = srv.serve(reqs)
+    srv.close()
+    assert sorted(r.rid for r in done) == list(range(len(reqs)))
+    assert srv.max_resident > 1
+    for r in done:
+        ref = serve.solve_alone(r, "fused")
+        assert np.array_equal(r.result["height"], ref["height"]), r.rid
