@@ -91,6 +91,12 @@ int swalbe_set_reference_height(swalbe_state *st, double href, double g);
  * with N = Lx*Ly (x9 for distributions).  Synchronous. */
 int swalbe_upload(swalbe_state *st, int32_t field, const void *host, size_t bytes);
 int swalbe_download(swalbe_state *st, int32_t field, void *host, size_t bytes);
+/* the stored representation (elements x 4 bytes for FP32 states, x 8 for
+ * FP64; FP32 heights and distributions as deviations from the reference
+ * film): an exact save / restore of a state, e.g. for a bit-exact restart */
+int swalbe_download_raw(swalbe_state *st, int32_t field, void *host, size_t bytes);
+int swalbe_upload_raw(swalbe_state *st, int32_t field, const void *host, size_t bytes);
+int swalbe_get_reference_height(swalbe_state *st, double *href, double *g);
 /* device-side copy of one field into another of the same kind (both
  * distributions, both heights or both plain planes): the broadcast
  * assignments of the drivers, e.g. `state.ftemp .= state.feq` */
@@ -144,10 +150,20 @@ typedef struct swalbe_step_opts {
     double *dh_series;     /* host [nsteps]: max(h) - min(h) before each step (:56), or NULL */
     int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) (measures.jl:13-17), or NULL */
     double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
-    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8 or 16 */
-    int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = collide at the source node,
-                              2 = collide at the destination node */
+    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8 or 16; variant 3:
+                              rows per strip segment, 0 = default (64), 4..4096 */
+    int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = LDS tiles, collide at the
+                              source node, 2 = LDS tiles, collide at the destination node,
+                              3 = LDS-free strips (one wavefront per 58-column strip) */
     int32_t nontemporal;   /* FP32 kernel: streaming (non-temporal) stores of the outputs */
+    int32_t pad_;
+    /* snapshot!(snap, height, t; dumping) (src/measures.jl:99-105): after every
+     * step t with t % snap_every == 0 the height is copied to snap_series
+     * (host, one Lx*Ly column-major block per snapshot, in step order), through
+     * a device ring and a second stream so the copies overlap later steps.
+     * 0 / NULL = off. */
+    int64_t snap_every;
+    double *snap_series;
 } swalbe_step_opts;
 int swalbe_time_loop(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
                      const swalbe_step_opts *opts);
@@ -172,7 +188,8 @@ int swalbe_init_random(swalbe_state *st, int32_t dist, uint64_t seed, double bas
  * with its parameters, an initial height and a step budget; it advances with
  * every swalbe_batch_step until the budget is spent and keeps its state
  * until removed.  Each film's arithmetic is the single-film FP32 kernel's,
- * so a film's result is the same alone, in any mix, and via swalbe_time_loop.
+ * so a film's result is the same alone, in any mix, and via swalbe_time_loop
+ * with opts.variant = 2, opts.tile_y = 16.
  * Initial height kinds follow src/initialvalues.jl (flat, randinterface!,
  * singledroplet, two_droplets); τ = 1 films keep one distribution buffer.
  * add / remove / step only enqueue work on the batch's stream (admissions

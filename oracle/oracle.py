@@ -75,6 +75,8 @@ def lib():
         L.orc_uniform.restype = C.c_double
         L.orc_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
         L.orc_normal_pair.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _D, _D]
+        L.orc_noise_index.restype = C.c_uint32
+        L.orc_noise_index.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
         L.orc_filmpressure_raw.restype = C.c_int
         L.orc_time_loop.restype = C.c_int
@@ -123,6 +125,11 @@ def normal_pair(seed, step, node, tag):
     a, b = C.c_double(), C.c_double()
     lib().orc_normal_pair(seed, step, node, tag, C.byref(a), C.byref(b))
     return a.value, b.value
+
+
+def noise_index(x, y, Lx):
+    """Stream index of node (x, y): vertical node pairs share one Philox call."""
+    return lib().orc_noise_index(int(x), int(y), int(Lx))
 
 
 def uniform(seed, step, node, tag):

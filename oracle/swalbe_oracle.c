@@ -136,9 +136,16 @@ double orc_cospi(double x) {
 }
 
 /* Counter layout: {c0, tag, step_lo, step_hi}, key {seed_lo, seed_hi}.
- * Normals: one call per node pair (c0 = node / 2); node parity b uses the
- * 32-bit uniforms (w + 1/2) 2^-32 of words 2b and 2b+1, then Box-Muller.
- * Uniforms: the node's own counter (c0 = node), 53 bits of words 0, 1. */
+ * Normals: stream index s; one call per index pair (c0 = s / 2); parity
+ * b = s % 2 uses the 32-bit uniforms (w + 1/2) 2^-32 of words 2b and 2b+1,
+ * then Box-Muller.  Lattice node (x, y) draws from stream index
+ * orc_noise_index(x, y, Lx) = 2 (x + Lx floor(y/2)) + y mod 2, so the two
+ * nodes of a vertical pair (x, 2m), (x, 2m+1) share one Philox call.
+ * Uniforms: the node's own counter (c0 = x + Lx y), 53 bits of words 0, 1. */
+uint32_t orc_noise_index(int x, int y, int Lx) {
+    return 2u * ((uint32_t)x + (uint32_t)Lx * (uint32_t)(y >> 1)) + (uint32_t)(y & 1);
+}
+
 static void philox_ctr(uint64_t seed, uint64_t step, uint32_t c0, uint32_t tag, uint32_t r[4]) {
     uint32_t ctr[4] = {c0, tag, (uint32_t)step, (uint32_t)(step >> 32)};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
@@ -419,7 +426,8 @@ void orc_thermal(double *kx, double *ky, const double *h, double kbt, double mu,
     const int64_t N = (int64_t)Lx * Ly;
     for (int64_t x = 0; x < N; ++x) {
         double z0, z1;
-        orc_normal_pair(seed, step, (uint32_t)x, 0u, &z0, &z1);
+        orc_normal_pair(seed, step, orc_noise_index((int)(x % Lx), (int)(x / Lx), Lx), 0u, &z0,
+                        &z1);
         kx[x] = z0;
         ky[x] = z1;
     }
@@ -465,7 +473,7 @@ void orc_init_normal(double *h, int Lx, int Ly, uint64_t seed, double base, doub
     const int64_t N = (int64_t)Lx * Ly;
     for (int64_t x = 0; x < N; ++x) {
         double z0, z1;
-        orc_normal_pair(seed, 0, (uint32_t)x, 1u, &z0, &z1);
+        orc_normal_pair(seed, 0, orc_noise_index((int)(x % Lx), (int)(x / Lx), Lx), 1u, &z0, &z1);
         h[x] = base + amp * z0;
     }
 }

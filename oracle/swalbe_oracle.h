@@ -63,6 +63,8 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 double orc_log(double x);
 void orc_sincospi(double x, double *s, double *c);
 double orc_cospi(double x);
+/* stream index of lattice node (x, y) for the normals: 2 (x + Lx floor(y/2)) + y mod 2 */
+uint32_t orc_noise_index(int x, int y, int Lx);
 void orc_normal_pair(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag,
                      double *z0, double *z1);
 double orc_uniform(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag);

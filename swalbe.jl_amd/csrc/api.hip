@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "swalbe_internal.h"
+#include "swalbe_batch.h"
 
 struct swalbe_state {
   swalbe::Film film;
@@ -225,8 +225,9 @@ int swalbe_create(int32_t Lx, int32_t Ly, int32_t dtype, int32_t kind, swalbe_st
     set_error("swalbe_create: invalid argument");
     return SWALBE_EINVAL;
   }
-  if ((int64_t)Lx * Ly > (int64_t)0xFFFFFFFFLL) {
-    set_error("swalbe_create: more than 2^32 nodes (noise counters are 32-bit per node)");
+  // noise stream indices 2 (x + Lx floor(y/2)) + y mod 2 are 32-bit
+  if ((int64_t)Lx * (Ly + 1) > (int64_t)0xFFFFFFFFLL) {
+    set_error("swalbe_create: Lx (Ly + 1) must stay below 2^32 (32-bit noise stream indices)");
     return SWALBE_EINVAL;
   }
   swalbe_state* st = new swalbe_state();
@@ -341,6 +342,61 @@ int swalbe_upload(swalbe_state* st, int32_t field, const void* host, size_t byte
   }
   SW_HIP(hipStreamSynchronize(st->stream));
   return copy_h2d(st, field, dst, tmp.data());
+}
+
+// Stored representation of a field (esz bytes per element; FP32 heights and
+// distributions as deviations from the reference film): exact round trips.
+int swalbe_download_raw(swalbe_state* st, int32_t field, void* host, size_t bytes) {
+  SW_CHECK_STATE(st);
+  if (field < 0 || field >= SWALBE_NFIELDS || !host) {
+    set_error("swalbe_download_raw: bad field id or null host pointer");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  int64_t count;
+  const void* src = field_ptr(st, field, &count);
+  if (!src) {
+    set_error("swalbe_download_raw: field not allocated");
+    return SWALBE_EINVAL;
+  }
+  if (bytes != (size_t)count * F.esz) {
+    set_error("swalbe_download_raw: byte count must be the stored size (elements x 4 or 8)");
+    return SWALBE_EINVAL;
+  }
+  SW_HIP(hipStreamSynchronize(st->stream));
+  return copy_d2h(st, field, host, src);
+}
+
+int swalbe_upload_raw(swalbe_state* st, int32_t field, const void* host, size_t bytes) {
+  SW_CHECK_STATE(st);
+  if (field < 0 || field >= SWALBE_NFIELDS || !host) {
+    set_error("swalbe_upload_raw: bad field id or null host pointer");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  Film& F = st->film;
+  const int64_t count = is_dist(field) ? 9 * F.N : F.N;
+  if (bytes != (size_t)count * F.esz) {
+    set_error("swalbe_upload_raw: byte count must be the stored size (elements x 4 or 8)");
+    return SWALBE_EINVAL;
+  }
+  if (field == SWALBE_THETA && !F.theta)
+    if (int rc = alloc_field(&F.theta, F.esz * F.N)) return rc;
+  if (field == SWALBE_THETA) F.kappa_valid = false;
+  if (field == SWALBE_FOUT || field == SWALBE_FTEMP) unalias_for_write(F, field);
+  int64_t c2;
+  void* dst = field_ptr(st, field, &c2);
+  SW_HIP(hipStreamSynchronize(st->stream));
+  return copy_h2d(st, field, dst, host);
+}
+
+// FP32 reference height and gravity of the deviation storage (href = 0 for FP64).
+int swalbe_get_reference_height(swalbe_state* st, double* href, double* g) {
+  SW_CHECK_STATE(st);
+  if (href) *href = st->film.dtype == SWALBE_F32 ? st->film.href : 0.0;
+  if (g) *g = st->film.g_enc;
+  return SWALBE_OK;
 }
 
 int swalbe_copy(swalbe_state* st, int32_t dst_field, int32_t src_field) {
@@ -713,6 +769,29 @@ __global__ void k_init_keys(unsigned long long* p, int64_t nsteps) {
   }
 }
 
+// Snapshots of the height: a device copy per snapshot into a ring slot on
+// the state's stream, then a device-to-host copy of that slot on a second
+// stream into pinned staging, decoded once the slot comes round again.
+struct SnapRing {
+  static constexpr int R = 4;
+  void* dev[R] = {};
+  void* host[R] = {};
+  hipEvent_t copied[R] = {};  // D2H of the slot done
+  hipEvent_t ready = nullptr;  // snapshot's device copy done (state stream)
+  hipStream_t cs = nullptr;
+  int64_t pending[R];          // snapshot index held by each slot, -1 = none
+  ~SnapRing() {
+    if (cs) hipStreamSynchronize(cs);
+    for (int r = 0; r < R; ++r) {
+      if (dev[r]) hipFree(dev[r]);
+      if (host[r]) hipHostFree(host[r]);
+      if (copied[r]) hipEventDestroy(copied[r]);
+    }
+    if (ready) hipEventDestroy(ready);
+    if (cs) hipStreamDestroy(cs);
+  }
+};
+
 struct LoopBuffers {
   unsigned long long* keys = nullptr;  // 2 per step
   unsigned long long* wet = nullptr;   // 1 per step
@@ -794,13 +873,14 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
     set_error("swalbe_time_loop: negative exponent");
     return SWALBE_EINVAL;
   }
-  const int tile_y = o->tile_y > 0 ? o->tile_y : 16;
-  if (tile_y != 8 && tile_y != 16) {
-    set_error("swalbe_time_loop: tile_y must be 8 or 16");
+  if (o->variant < 0 || o->variant > 3) {
+    set_error("swalbe_time_loop: variant must be 0, 1, 2 or 3");
     return SWALBE_EINVAL;
   }
-  if (o->variant < 0 || o->variant > 2) {
-    set_error("swalbe_time_loop: variant must be 0, 1 or 2");
+  const bool strips = o->variant == 3;
+  const int tile_y = o->tile_y > 0 ? o->tile_y : (strips ? kStripSeg : 16);
+  if (strips ? (tile_y < 4 || tile_y > 4096) : (tile_y != 8 && tile_y != 16)) {
+    set_error("swalbe_time_loop: tile_y must be 8 or 16 (4..4096 rows per segment for variant 3)");
     return SWALBE_EINVAL;
   }
   const StepParams sp = make_step_params(p, o, 0);
@@ -820,6 +900,39 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
   if (want_dh)  // even slots hold max keys (start 0), odd slots min keys (start ~0)
     hipLaunchKernelGGL(k_init_keys, dim3(64), dim3(256), 0, st->stream, lb.keys, nsteps);
   if (want_wet) hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, st->stream, lb.wet, 0ull, nsteps);
+  const bool want_snap = o->snap_every > 0 && o->snap_series != nullptr;
+  if (ms_out && want_snap) {
+    set_error("swalbe_time_loop_timed: snapshots are not taken by the timed variant");
+    return SWALBE_EINVAL;
+  }
+  SnapRing ring;
+  const size_t plane_bytes = (size_t)F.N * F.esz;
+  const double snap_off = enc_offset(F, SWALBE_HEIGHT, 0);
+  auto snap_drain = [&](int r) -> int {  // decode the snapshot held by slot r
+    if (ring.pending[r] < 0) return SWALBE_OK;
+    SW_HIP(hipEventSynchronize(ring.copied[r]));
+    double* dst = o->snap_series + ring.pending[r] * F.N;
+    if (F.esz == 8) {
+      const double* src = (const double*)ring.host[r];
+      for (int64_t i = 0; i < F.N; ++i) dst[i] = src[i] + snap_off;
+    } else {
+      const float* src = (const float*)ring.host[r];
+      for (int64_t i = 0; i < F.N; ++i) dst[i] = (double)src[i] + snap_off;
+    }
+    ring.pending[r] = -1;
+    return SWALBE_OK;
+  };
+  if (want_snap) {
+    for (int r = 0; r < SnapRing::R; ++r) {
+      ring.pending[r] = -1;
+      SW_HIP(hipMalloc(&ring.dev[r], plane_bytes));
+      SW_HIP(hipHostMalloc(&ring.host[r], plane_bytes, hipHostMallocDefault));
+      SW_HIP(hipEventCreateWithFlags(&ring.copied[r], hipEventDisableTiming));
+    }
+    SW_HIP(hipEventCreateWithFlags(&ring.ready, hipEventDisableTiming));
+    SW_HIP(hipStreamCreateWithFlags(&ring.cs, hipStreamNonBlocking));
+  }
+  int64_t ksnap = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ms_out) {
     SW_HIP(hipEventCreate(&e0));
@@ -840,7 +953,20 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
       enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal);
     else
       enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal);
+    if (want_snap && t % o->snap_every == 0) {
+      const int r = (int)(ksnap % SnapRing::R);
+      if (int rc = snap_drain(r)) return rc;
+      SW_HIP(hipMemcpyAsync(ring.dev[r], F.height(), plane_bytes, hipMemcpyDeviceToDevice,
+                            st->stream));
+      SW_HIP(hipEventRecord(ring.ready, st->stream));
+      SW_HIP(hipStreamWaitEvent(ring.cs, ring.ready, 0));
+      SW_HIP(hipMemcpyAsync(ring.host[r], ring.dev[r], plane_bytes, hipMemcpyDeviceToHost, ring.cs));
+      SW_HIP(hipEventRecord(ring.copied[r], ring.cs));
+      ring.pending[r] = ksnap++;
+    }
   }
+  for (int r = 0; r < SnapRing::R && want_snap; ++r)
+    if (int rc = snap_drain((int)((ksnap + r) % SnapRing::R))) return rc;
   if (ms_out) {
     SW_HIP(hipEventRecord(e1, st->stream));
     SW_HIP(hipEventSynchronize(e1));

@@ -42,7 +42,7 @@ __global__ void k_film_init(T* __restrict__ h, int Lx, int Ly, swalbe_film_init 
     double v = in.base;
     if (in.kind == SWALBE_INIT_NORMAL) {
       double z0, z1;
-      normal_pair(in.seed, 0, (uint32_t)x, kTagInitNormal, &z0, &z1);
+      normal_pair(in.seed, 0, noise_index(i, j, Lx), kTagInitNormal, &z0, &z1);
       v = in.base + in.amp * z0;
     } else if (in.kind == SWALBE_INIT_CAP || in.kind == SWALBE_INIT_TWO_CAPS) {
       // spherical cap (cos(asin(r/R)) - cos πθ) R inside r <= R, else the
@@ -377,7 +377,8 @@ int swalbe_batch_add(swalbe_batch* b, const swalbe_params* p, int32_t Lx, int32_
                      const swalbe_film_init* init, int64_t nsteps, int32_t* slot_out) {
   if (!b || !p || !init || !slot_out || Lx < 1 || Ly < 1 || nsteps < 0)
     return bad("swalbe_batch_add: bad arguments");
-  if ((int64_t)Lx * Ly > (int64_t)0xFFFFFFFFLL) return bad("swalbe_batch_add: film too large");
+  if ((int64_t)Lx * (Ly + 1) > (int64_t)0xFFFFFFFFLL)  // 32-bit noise stream indices
+    return bad("swalbe_batch_add: film too large");
   if (p->n < 0 || p->m < 0) return bad("swalbe_batch_add: negative exponent");
   hipSetDevice(b->device);
   int slot = -1;
@@ -419,10 +420,17 @@ int swalbe_batch_add(swalbe_batch* b, const swalbe_params* p, int32_t Lx, int32_
   d.f[1] = two_f ? (float*)(base + 7 * plane + 9 * plane) : nullptr;
   d.Lx = Lx;
   d.Ly = Ly;
-  d.ntx = (Lx + 63) / 64;
-  d.ntiles = d.ntx * ((Ly + kBatchTileY - 1) / kBatchTileY);
-  d.small = film_is_small(Lx, Ly, kBatchTileY);
-  d.pairs = (!d.small && (Lx % 2) == 0) ? 1 : 0;
+  if (kBatchStrips) {
+    d.ntx = (Lx + kStripOut - 1) / kStripOut;
+    d.ntiles = d.ntx * ((Ly + kStripSeg - 1) / kStripSeg);
+    d.small = strip_is_small(Lx, Ly);
+    d.pairs = 0;
+  } else {
+    d.ntx = (Lx + 63) / 64;
+    d.ntiles = d.ntx * ((Ly + kBatchTileY - 1) / kBatchTileY);
+    d.small = film_is_small(Lx, Ly, kBatchTileY);
+    d.pairs = (!d.small && (Ly % 2) == 0) ? 1 : 0;
+  }
   d.read_ft = two_f ? 1 : 0;
   d.thermal = init->thermal ? 1 : 0;
   d.body = init->body ? 1 : 0;

@@ -256,11 +256,11 @@ __global__ void k_slippage(T* __restrict__ sx, T* __restrict__ sy, const T* __re
 template <typename T>
 __global__ void k_thermal(T* __restrict__ kx, T* __restrict__ ky, const T* __restrict__ h,
                           T kb26, T delta, T d3dd, uint64_t seed, uint64_t step, int64_t N,
-                          Enc e) {
+                          int Lx, Enc e) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < N;
        x += (int64_t)gridDim.x * blockDim.x) {
     double z0, z1;
-    normal_pair(seed, step, (uint32_t)x, kTagThermal, &z0, &z1);
+    normal_pair(seed, step, noise_index((int)(x % Lx), (int)(x / Lx), Lx), kTagThermal, &z0, &z1);
     const T hh = dec_h(h[x], e);
     const T amp = sqrt(kb26 * hh / (((T(2) * hh) * hh + (T(6) * hh) * delta) + d3dd));
     kx[x] = (T)z0 * amp;
@@ -316,7 +316,7 @@ __global__ void k_fill(T* __restrict__ p, T v, int64_t N) {
 
 template <typename T>
 __global__ void k_init_random(T* __restrict__ h, int dist, uint64_t seed, double base,
-                              double amp, double href, int64_t N) {
+                              double amp, double href, int64_t N, int Lx) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < N;
        x += (int64_t)gridDim.x * blockDim.x) {
     double v;
@@ -324,7 +324,8 @@ __global__ void k_init_random(T* __restrict__ h, int dist, uint64_t seed, double
       v = base + amp * uniform(seed, 0, (uint32_t)x, kTagInitUniform);
     } else {
       double z0, z1;
-      normal_pair(seed, 0, (uint32_t)x, kTagInitNormal, &z0, &z1);
+      normal_pair(seed, 0, noise_index((int)(x % Lx), (int)(x / Lx), Lx), kTagInitNormal, &z0,
+                  &z1);
       v = base + amp * z0;
     }
     h[x] = (T)(v - href);
@@ -390,7 +391,7 @@ void launch_thermal(T* kx, T* ky, const T* h, double kbt, double mu, double delt
   const int64_t N = (int64_t)Lx * Ly;
   hipLaunchKernelGGL(k_thermal<T>, grid1d(N), dim3(kBlock1D), 0, s, kx, ky, h,
                      (T)(((2.0 * kbt) * mu) * 6.0), (T)delta, (T)((3.0 * delta) * delta), seed,
-                     step, N, enc);
+                     step, N, Lx, enc);
 }
 template <typename T>
 void launch_inclination(T* Fx, T* Fy, const T* h, double ax, double ay, double fac, int Lx,
@@ -414,7 +415,7 @@ void launch_init_random(T* h, int dist, uint64_t seed, double base, double amp, 
                         int Lx, int Ly, hipStream_t s) {
   const int64_t N = (int64_t)Lx * Ly;
   hipLaunchKernelGGL(k_init_random<T>, grid1d(N), dim3(kBlock1D), 0, s, h, dist, seed, base, amp,
-                     href, N);
+                     href, N, Lx);
 }
 
 #define SW_INSTANTIATE(T)                                                                        \

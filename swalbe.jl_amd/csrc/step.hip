@@ -202,24 +202,25 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
       sh_fx[idx] = Fx;
       sh_fy[idx] = Fy;
     };
-    if (!small && (Lx & 1) == 0) {
-      // one Philox call per pair of x-adjacent nodes: 34 pair columns cover
-      // x0-2 .. x0+65, tile column c = 2 pc - 1 + b
-      constexpr int NPC = (TX + 4) / 2;
-      for (int t = tid; t < UR * NPC; t += NT) {
-        const int r = t / NPC, pc = t - r * NPC;
+    if (!small && (Ly & 1) == 0) {
+      // one Philox call per vertical node pair (x, 2m), (x, 2m+1): pair row
+      // q covers halo rows 2q-1 (even y) and 2q (odd y); y0 is even
+      constexpr int NPR = TY / 2 + 2;
+      for (int t = tid; t < NPR * UC; t += NT) {
+        const int q = t / UC, c = t - q * UC;
+        const int re = 2 * q - 1, ro = 2 * q;
         double z[4] = {0.0, 0.0, 0.0, 0.0};
         if constexpr (THERMAL) {
-          const int gxx = wrap1(x0 - 2 + 2 * pc, Lx);
-          const int64_t node = gxx + (int64_t)Lx * wrap1(y0 - 1 + r, Ly);
+          const int gxx = wrap1(x0 - 1 + c, Lx);
+          const int ye = wrap1(y0 - 2 + 2 * q, Ly);
           uint32_t w[4];
-          philox_ctr(sp.seed, step, (uint32_t)(node >> 1), kTagThermal, w);
+          philox_ctr(sp.seed, step, (uint32_t)gxx + (uint32_t)Lx * (uint32_t)(ye >> 1), kTagThermal,
+                     w);
           box_muller(u32(w[0]), u32(w[1]), &z[0], &z[1]);
           box_muller(u32(w[2]), u32(w[3]), &z[2], &z[3]);
         }
-        const int c0 = 2 * pc - 1;
-        if (c0 >= 0) fnode(r, c0, z[0], z[1]);
-        if (c0 + 1 < UC) fnode(r, c0 + 1, z[2], z[3]);
+        if (re >= 0) fnode(re, c, z[0], z[1]);
+        if (ro < UR) fnode(ro, c, z[2], z[3]);
       }
     } else {
       for (int idx = tid; idx < UR * UC; idx += NT) {
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
           int gxx = x0 - 1 + c, gyy = y0 - 1 + r;
           gxx = small ? wrapm(gxx, Lx) : wrap1(gxx, Lx);
           gyy = small ? wrapm(gyy, Ly) : wrap1(gyy, Ly);
-          normal_pair(sp.seed, step, (uint32_t)(gxx + (int64_t)Lx * gyy), kTagThermal, &z0, &z1);
+          normal_pair(sp.seed, step, noise_index(gxx, gyy, Lx), kTagThermal, &z0, &z1);
         }
         fnode(r, c, z0, z1);
       }

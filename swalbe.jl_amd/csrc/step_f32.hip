@@ -234,32 +234,34 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
     }
   };
   if (R.pairs) {
-    // 34 pair columns cover global x0-2 .. x0+65; tile column c = 2 pc - 1 + b
-    constexpr int NPC = (TX + 4) / 2;
-    for (int t = tid; t < UR * NPC; t += NT) {
-      const int r = t / NPC, pc = t - r * NPC;
+    // one Philox call per vertical node pair (x, 2m), (x, 2m+1): pair row q
+    // covers halo rows 2q-1 (even y) and 2q (odd y); y0 is even
+    constexpr int NPR = TY / 2 + 2;
+    for (int t = tid; t < NPR * UC; t += NT) {
+      const int q = t / UC, c = t - q * UC;
+      const int re = 2 * q - 1, ro = 2 * q;
       float z[4] = {0.f, 0.f, 0.f, 0.f};
       if constexpr (THERMAL) {
-        const int gxp = wx(x0 - 2 + 2 * pc);
-        const int64_t node = gxp + (int64_t)Lx * wy(y0 - 1 + r);
+        const int gxx = wrap1(x0 - 1 + c, Lx);
+        const int ye = wrap1(y0 - 2 + 2 * q, Ly);
         uint32_t w[4];
-        philox_ctr(R.seed, R.step, (uint32_t)(node >> 1), kTagThermal, w);
+        philox_ctr(R.seed, R.step, (uint32_t)gxx + (uint32_t)Lx * (uint32_t)(ye >> 1), kTagThermal,
+                   w);
         box_muller_f32(w[0], w[1], &z[0], &z[1]);
         box_muller_f32(w[2], w[3], &z[2], &z[3]);
       }
-      const int c0 = 2 * pc - 1;
-      if (c0 >= 0) fnode(r, c0, z[0], z[1]);
-      if (c0 + 1 < UC) fnode(r, c0 + 1, z[2], z[3]);
+      if (re >= 0) fnode(re, c, z[0], z[1]);
+      if (ro < UR) fnode(ro, c, z[2], z[3]);
     }
   } else {
     for (int t = tid; t < UN; t += NT) {
       const int r = t / UC, c = t - r * UC;
       float z0 = 0.f, z1 = 0.f;
       if constexpr (THERMAL) {
-        const int64_t node = wx(x0 - 1 + c) + (int64_t)Lx * wy(y0 - 1 + r);
+        const uint32_t s = noise_index(wx(x0 - 1 + c), wy(y0 - 1 + r), Lx);
         uint32_t w[4];
-        philox_ctr(R.seed, R.step, (uint32_t)(node >> 1), kTagThermal, w);
-        const int b = (int)(node & 1);
+        philox_ctr(R.seed, R.step, s >> 1, kTagThermal, w);
+        const int b = (int)(s & 1u);
         box_muller_f32(w[2 * b], w[2 * b + 1], &z0, &z1);
       }
       fnode(r, c, z0, z1);
@@ -325,6 +327,243 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
   }
 }
 
+
+// ---------------------------------------------------------------- strips
+// LDS-free variant: one wavefront owns a strip of 64 columns (x0-3 .. x0+60,
+// one per lane) and marches down a segment of rows.  Per row it loads h
+// three rows ahead, computes p two rows ahead and the post-collision
+// populations one row ahead, so the 3-row halo of h, p and f lives in
+// registers, x-neighbours come from DPP wave shifts and no barrier is needed.
+// The halo narrows by one column per stencil: 58 columns (lanes 3..60) are
+// written per strip.
+// Value held by the lane of column x-1 (wave_shr:1) / x+1 (wave_shl:1); the
+// edge lanes keep their own value (they are halo lanes, never written).
+__device__ __forceinline__ float at_xm1(float v) {
+  const int i = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(i, i, 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float at_xp1(float v) {
+  const int i = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(i, i, 0x130, 0xF, 0xF, false));
+}
+
+struct Row3 {
+  float v, m, p;  // value at x, x-1, x+1
+};
+
+__device__ __forceinline__ Row3 row3(float v) { return Row3{v, at_xm1(v), at_xp1(v)}; }
+
+// Rows [y0, y0 + S) of the strip whose first output column is x0.
+template <bool READ_FT, bool THERMAL, bool SERIES>
+__device__ __forceinline__ void strip_run(const TileIO& io, const F32K& K, const TileRun& R,
+                                          int x0, int y0, int S, SeriesAcc& acc) {
+#pragma clang fp contract(fast)
+  const int lane = threadIdx.x & 63;
+  const int Lx = R.Lx, Ly = R.Ly;
+  const int64_t FS = R.fstride;
+  const int small = R.small;
+  auto wy = [&](int a) { return small ? wrapm(a, Ly) : wrap1(a, Ly); };
+  const int xc = x0 - 3 + lane;  // this lane's column (unwrapped)
+  const int gx = small ? wrapm(xc, Lx) : wrap1(xc, Lx);
+  const bool out_lane = lane >= 3 && lane < 3 + kStripOut && xc < Lx;
+  const float w1 = 1.0f / 9.0f, w5 = 1.0f / 36.0f;
+  const int jlast = y0 + S - 1;
+
+  // prefetched inputs of the next iteration: h(j+3), u(j+1), κ(j+2), f_t(j+1)
+  float nh, nu, nv, nk = 0.f, nft[kQ];
+  auto prefetch = [&](int j) {
+    const int64_t r3 = (int64_t)Lx * wy(j + 3) + gx, r1 = (int64_t)Lx * wy(j + 1) + gx;
+    nh = io.h_in[r3];
+    nu = io.ux_in[r1];
+    nv = io.uy_in[r1];
+    if (R.use_kappa) nk = io.kappa[(int64_t)Lx * wy(j + 2) + gx];
+    if constexpr (READ_FT) {
+#pragma unroll
+      for (int k = 0; k < kQ; ++k) nft[k] = io.ft[k * FS + r1];
+    }
+  };
+
+  uint32_t wc2 = 0, wc3 = 0;  // cached noise words of the odd row of a pair
+
+  // one row: j is the output row; the arrays rotate roles between calls
+  auto body = [&](int j, Row3& hN, Row3& h1, Row3& h2, Row3& pN, Row3& p0, Row3& p1, float* fN,
+                  float* fm, float* f0) {
+    if constexpr (SERIES) {
+      if (j >= y0 && out_lane) {  // h(j) is the step's input height at the output row
+        const double hv = (double)K.href + (double)hN.v;
+        acc.lmax = hv > acc.lmax ? hv : acc.lmax;
+        acc.lmin = hv < acc.lmin ? hv : acc.lmin;
+        acc.cnt += hv > R.wet_thresh ? 1u : 0u;
+      }
+    }
+    const float u = nu, v = nv, kap = R.use_kappa ? nk : K.kap;
+    float ft[kQ];
+    if constexpr (READ_FT) {
+#pragma unroll
+      for (int k = 0; k < kQ; ++k) ft[k] = nft[k];
+    }
+    hN = row3(nh);  // h(j+3)
+    if (j < jlast) prefetch(j + 1);
+
+    // p(j+2) from h rows j+1 .. j+3
+    {
+      const float L = (2.0f / 3.0f) * (((h1.v + h2.m) + h2.p) + hN.v) +
+                      (1.0f / 6.0f) * (((h1.m + h1.p) + hN.p) + hN.m) - (10.0f / 3.0f) * h2.v;
+      const float hh = K.href + h2.v;
+      const float a = K.hmin * __builtin_amdgcn_rcpf(hh + K.hcrit);
+      pN = row3(-K.gamma * (kap * powlaw_f(a, K.n, K.m, K.powmode)) - K.gamma * L);
+    }
+    if (j >= y0 - 2) {
+      // F(j+1) from p rows j .. j+2, then the nine post-collision values
+      const float dh = h1.v, hh = K.href + dh;
+      const float gxp = (-1.0f / 3.0f) * (p1.m - p1.p) -
+                        (1.0f / 12.0f) * (((p0.m - p0.p) - pN.p) + pN.m);
+      const float gyp = (-1.0f / 3.0f) * (p0.v - pN.v) -
+                        (1.0f / 12.0f) * (((p0.m + p0.p) - pN.p) - pN.m);
+      const float iden = __builtin_amdgcn_rcpf((2.0f * hh + K.c6d) * hh + K.c3dd);
+      const float sl = K.c6mu * hh * iden;
+      float Fx = -hh * gxp - sl * u;
+      float Fy = -hh * gyp - sl * v;
+      if constexpr (THERMAL) {
+        // one Philox call per vertical node pair (x, 2m), (x, 2m+1)
+        const int rw = wy(j + 1);
+        uint32_t wa, wb;
+        if ((rw & 1) == 0 || j == y0 - 2) {
+          uint32_t w[4];
+          philox_ctr(R.seed, R.step, (uint32_t)gx + (uint32_t)Lx * (uint32_t)(rw >> 1),
+                     kTagThermal, w);
+          wa = (rw & 1) ? w[2] : w[0];
+          wb = (rw & 1) ? w[3] : w[1];
+          wc2 = w[2];
+          wc3 = w[3];
+        } else {
+          wa = wc2;
+          wb = wc3;
+        }
+        float z0, z1;
+        box_muller_f32(wa, wb, &z0, &z1);
+        const float amp = __builtin_amdgcn_sqrtf(K.kb26 * hh * iden);
+        Fx -= z0 * amp;
+        Fy -= z1 * amp;
+      }
+      if (R.body) {
+        Fx += hh * K.bfx;
+        Fy += hh * K.bfy;
+      }
+      const float usq = u * u + v * v;
+      const float gg = dh * (hh + K.href);
+      const float w1h = w1 * hh, w5h = w5 * hh;
+      const float b1 = w1 * K.g0 * gg - 1.5f * w1h * usq;
+      const float b5 = w5 * K.g0 * gg - 1.5f * w5h * usq;
+      const float a5 = u + v, a6 = v - u;
+      const float c3 = 1.0f / 3.0f, c24 = 1.0f / 24.0f;
+      float fe[kQ], Sk[kQ];
+      fe[0] = (dh - K.c56g * gg) - (2.0f / 3.0f) * hh * usq;
+      fe[1] = b1 + w1h * ((4.5f * u + 3.0f) * u);
+      fe[2] = b1 + w1h * ((4.5f * v + 3.0f) * v);
+      fe[3] = b1 + w1h * ((4.5f * u - 3.0f) * u);
+      fe[4] = b1 + w1h * ((4.5f * v - 3.0f) * v);
+      fe[5] = b5 + w5h * ((4.5f * a5 + 3.0f) * a5);
+      fe[6] = b5 + w5h * ((4.5f * a6 + 3.0f) * a6);
+      fe[7] = b5 + w5h * ((4.5f * a5 - 3.0f) * a5);
+      fe[8] = b5 + w5h * ((4.5f * a6 - 3.0f) * a6);
+      Sk[0] = 0.0f;
+      Sk[1] = c3 * Fx;
+      Sk[2] = c3 * Fy;
+      Sk[3] = -c3 * Fx;
+      Sk[4] = -c3 * Fy;
+      Sk[5] = c24 * (Fx + Fy);
+      Sk[6] = c24 * (Fy - Fx);
+      Sk[7] = -c24 * (Fx + Fy);
+      Sk[8] = c24 * (Fx - Fy);
+#pragma unroll
+      for (int k = 0; k < kQ; ++k) {
+        float f = READ_FT ? K.itau * fe[k] + Sk[k] : fe[k] + Sk[k];
+        if constexpr (READ_FT) f += K.omeg * ft[k];
+        fN[k] = f;
+      }
+    }
+    if (j >= y0) {
+      // pull stream: f_k(x, j) = fo_k(x - c_k) from rows j-1 (fm), j (f0), j+1 (fN)
+      float fo[kQ];
+      fo[0] = f0[0];
+      fo[1] = at_xm1(f0[1]);
+      fo[2] = fm[2];
+      fo[3] = at_xp1(f0[3]);
+      fo[4] = fN[4];
+      fo[5] = at_xm1(fm[5]);
+      fo[6] = at_xp1(fm[6]);
+      fo[7] = at_xp1(fN[7]);
+      fo[8] = at_xm1(fN[8]);
+      if (out_lane) {
+        const int64_t xo = (int64_t)Lx * j + xc;
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) store_f(&io.f_out[k * FS + xo], fo[k], R.nts);
+        const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) +
+                        fo[8];
+        const float ih = __builtin_amdgcn_rcpf(K.href + s);
+        store_f(&io.h_out[xo], s, R.nts);
+        store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, R.nts);
+        store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, R.nts);
+        store_f(&io.p_out[xo], p0.v, R.nts);
+      }
+    }
+  };
+
+  // prologue: h rows y0-3, y0-2 and the prefetch of iteration y0-4
+  Row3 hA{0.f, 0.f, 0.f};  // role h(j) at j = y0-4: overwritten before use
+  Row3 hB = row3(io.h_in[(int64_t)Lx * wy(y0 - 3) + gx]);
+  Row3 hC = row3(io.h_in[(int64_t)Lx * wy(y0 - 2) + gx]);
+  prefetch(y0 - 4);
+  Row3 pA{0.f, 0.f, 0.f}, pB{0.f, 0.f, 0.f}, pC{0.f, 0.f, 0.f};
+  float fA[kQ], fB[kQ], fC[kQ];
+#pragma unroll
+  for (int k = 0; k < kQ; ++k) fA[k] = fB[k] = fC[k] = 0.f;
+  // roles per call: (hN, h1, h2) = (h(j), h(j+1), h(j+2)) on entry;
+  // (pN, p0, p1) = (p(j-1), p(j), p(j+1)); (fN, fm, f0) = (free, fo(j-1), fo(j))
+  int j = y0 - 4;
+  for (; j + 2 <= jlast; j += 3) {
+    body(j, hA, hB, hC, pA, pB, pC, fA, fB, fC);
+    body(j + 1, hB, hC, hA, pB, pC, pA, fB, fC, fA);
+    body(j + 2, hC, hA, hB, pC, pA, pB, fC, fA, fB);
+  }
+  if (j <= jlast) body(j, hA, hB, hC, pA, pB, pC, fA, fB, fC);
+  if (j + 1 <= jlast) body(j + 1, hB, hC, hA, pB, pC, pA, fB, fC, fA);
+}
+
+// Single film: one wavefront per (strip, segment) work item, items in
+// segment-major order, each XCD taking a contiguous share.
+template <bool READ_FT, bool THERMAL>
+__global__ __launch_bounds__(64) void k_strip_f32(TileIO io, F32K K, TileRun R, int nstrips,
+                                                  int seg, int items, int chunk, StepSeries ser) {
+  const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+  const int item = xcd * chunk + k;
+  if (k >= chunk || item >= items) return;
+  const int sg = item / nstrips, st = item - sg * nstrips;
+  const int y0 = sg * seg;
+  const int S = min(seg, R.Ly - y0);
+  SeriesAcc acc{-__builtin_inf(), __builtin_inf(), 0u};
+  const bool series = ser.dh_keys || ser.wet;
+  if (series) strip_run<READ_FT, THERMAL, true>(io, K, R, st * kStripOut, y0, S, acc);
+  else strip_run<READ_FT, THERMAL, false>(io, K, R, st * kStripOut, y0, S, acc);
+  if (!series) return;
+  unsigned long long kmax = dkey(acc.lmax), kmin = dkey(acc.lmin);
+  unsigned int lcnt = acc.cnt;
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long a = __shfl_xor(kmax, o), b = __shfl_xor(kmin, o);
+    kmax = a > kmax ? a : kmax;
+    kmin = b < kmin ? b : kmin;
+    lcnt += __shfl_xor(lcnt, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (ser.dh_keys) {
+      atomicMax(&ser.dh_keys[0], kmax);
+      atomicMin(&ser.dh_keys[1], kmin);
+    }
+    if (ser.wet) atomicAdd(&ser.wet[0], (unsigned long long)lcnt);
+  }
+}
+
 // ---------------------------------------------------------------- single film
 template <int TY, bool READ_FT, bool THERMAL, bool SRC>
 __global__ __launch_bounds__(NT) void k_step_f32(TileIO io, F32K K, TileRun R, int ntx,
@@ -382,7 +621,7 @@ void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser
   R.Lx = Lx;
   R.Ly = Ly;
   R.small = film_is_small(Lx, Ly, TY);
-  R.pairs = (!R.small && (Lx % 2) == 0) ? 1 : 0;
+  R.pairs = (!R.small && (Ly % 2) == 0) ? 1 : 0;
   R.nts = nts;
   R.use_kappa = io.kappa != nullptr;
   R.body = sp.body;
@@ -419,16 +658,29 @@ __global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ 
   __shared__ Smem<TY, false> sm;
   const int total = hdr->total_tiles, nf = hdr->nfilms;
   const unsigned long long L = *counter;
+  // each workgroup takes a contiguous run of its XCD's share: consecutive
+  // tiles of one film, whose halos the previous tile left in L2, and one
+  // film lookup per film change instead of one per tile
   const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3, nj = gridDim.x >> 3;
   const int chunk = (total + 7) >> 3;
-  const int tend = min(xcd * chunk + chunk, total);
+  const int per = (chunk + nj - 1) / nj;
+  const int tbeg = xcd * chunk + j * per;
+  const int tend = min(min(xcd * chunk + chunk, tbeg + per), total);
   SeriesAcc acc{0.0, 0.0, 0u};
-  for (int tile = xcd * chunk + j; tile < tend; tile += nj) {
-    int lo = 0, hi = nf - 1;
+  int lo = 0, nxt = 0;
+  if (tbeg < tend) {
+    int hi = nf - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (tstart[mid] <= tile) lo = mid;
+      if (tstart[mid] <= tbeg) lo = mid;
       else hi = mid - 1;
+    }
+    nxt = lo + 1 < nf ? tstart[lo + 1] : total;
+  }
+  for (int tile = tbeg; tile < tend; ++tile) {
+    while (tile >= nxt) {  // next film
+      ++lo;
+      nxt = lo + 1 < nf ? tstart[lo + 1] : total;
     }
     const BatchFilm& F = films[lo];
     if (L < F.admit || L >= F.end) continue;
@@ -476,6 +728,73 @@ __global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ 
   }
 }
 
+// Batch of films on strips: one wavefront per (film, strip, segment) item.
+// The fixed grid strides over each XCD's contiguous share of the items.
+__global__ __launch_bounds__(64) void k_batch_strip_f32(const BatchFilm* __restrict__ films,
+                                                      const int* __restrict__ tstart,
+                                                      const BatchHdr* __restrict__ hdr,
+                                                      const unsigned long long* __restrict__ counter,
+                                                      int nts) {
+  const int total = hdr->total_tiles, nf = hdr->nfilms;
+  const unsigned long long L = *counter;
+  const int xcd = blockIdx.x & 7, nj = gridDim.x >> 3;
+  const int chunk = (total + 7) >> 3;
+  const int tend = min(xcd * chunk + chunk, total);
+  SeriesAcc acc{0.0, 0.0, 0u};
+  for (int item = xcd * chunk + (blockIdx.x >> 3); item < tend; item += nj) {
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tstart[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchFilm& F = films[lo];
+    if (L < F.admit || L >= F.end) continue;
+    const unsigned long long n = L - F.admit;
+    const int cur = (int)(n & 1ull);
+    TileIO io;
+    io.h_in = F.h[cur];
+    io.ux_in = F.ux[cur];
+    io.uy_in = F.uy[cur];
+    io.h_out = F.h[cur ^ 1];
+    io.ux_out = F.ux[cur ^ 1];
+    io.uy_out = F.uy[cur ^ 1];
+    io.p_out = F.p;
+    io.kappa = nullptr;
+    if (F.read_ft) {
+      io.ft = F.f[cur];
+      io.f_out = F.f[cur ^ 1];
+    } else {
+      io.ft = nullptr;
+      io.f_out = F.f[0];
+    }
+    TileRun R;
+    R.Lx = F.Lx;
+    R.Ly = F.Ly;
+    R.small = F.small;
+    R.pairs = 0;
+    R.nts = nts;
+    R.use_kappa = 0;
+    R.body = F.body;
+    R.fstride = F.fstride;
+    R.seed = F.seed;
+    R.step = (uint64_t)F.t0 + n + 1;
+    R.wet_thresh = 0.0;
+    const int local = item - tstart[lo];
+    const int sg = local / F.ntx, st = local - sg * F.ntx;
+    const int y0 = sg * kStripSeg;
+    const int S = min(kStripSeg, F.Ly - y0);
+    const F32K K = F.K;
+    if (F.read_ft) {
+      if (F.thermal) strip_run<true, true, false>(io, K, R, st * kStripOut, y0, S, acc);
+      else strip_run<true, false, false>(io, K, R, st * kStripOut, y0, S, acc);
+    } else {
+      if (F.thermal) strip_run<false, true, false>(io, K, R, st * kStripOut, y0, S, acc);
+      else strip_run<false, false, false>(io, K, R, st * kStripOut, y0, S, acc);
+    }
+  }
+}
+
 __global__ void k_batch_advance(unsigned long long* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter += 1ull;
 }
@@ -512,11 +831,48 @@ int film_is_small(int Lx, int Ly, int tile_y) {
 
 // variant: 0 = default (collide at the destination), 1 = collide at the
 // source node, 2 = collide at the destination.  nts: non-temporal stores.
+void launch_strip(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
+                  int Lx, int Ly, int64_t fstride, Enc enc, int seg, int nts, hipStream_t s) {
+  TileIO t{io.h_in, io.ux_in, io.uy_in, io.ft, io.f_out, io.h_out, io.ux_out, io.uy_out,
+           io.p_out, io.kappa};
+  TileRun R;
+  R.Lx = Lx;
+  R.Ly = Ly;
+  R.small = strip_is_small(Lx, Ly);
+  R.pairs = 0;
+  R.nts = nts;
+  R.use_kappa = io.kappa != nullptr;
+  R.body = sp.body;
+  R.fstride = fstride;
+  R.seed = sp.seed;
+  R.step = step;
+  R.wet_thresh = sp.wet_thresh;
+  const F32K K = make_f32k(sp, enc);
+  const int nstrips = (Lx + kStripOut - 1) / kStripOut, nseg = (Ly + seg - 1) / seg;
+  const int items = nstrips * nseg, chunk = (items + 7) / 8;
+  const dim3 grid(8 * chunk), block(64);
+#define SW_L(RF, TH)                                                                           \
+  hipLaunchKernelGGL((k_strip_f32<RF, TH>), grid, block, 0, s, t, K, R, nstrips, seg, items, chunk, \
+                     ser)
+  if (sp.read_ft) {
+    if (sp.thermal) SW_L(true, true);
+    else SW_L(true, false);
+  } else {
+    if (sp.thermal) SW_L(false, true);
+    else SW_L(false, false);
+  }
+#undef SW_L
+}
+
+int strip_is_small(int Lx, int Ly) { return (Lx < 64 || Ly < 8) ? 1 : 0; }
+
 void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
                      int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, int variant, int nts,
                      hipStream_t s) {
   const bool t8 = tile_y == 8;
-  if (variant == 1) {
+  if (variant == 3) {
+    launch_strip(io, sp, ser, step, Lx, Ly, fstride, enc, tile_y > 0 ? tile_y : kStripSeg, nts, s);
+  } else if (variant == 1) {
     if (t8) launch_single<8, true>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
     else launch_single<16, true>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
   } else {
@@ -527,17 +883,22 @@ void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries s
 
 void launch_batch_f32(const BatchFilm* films, const int* tstart, const BatchHdr* hdr,
                       unsigned long long* counter, int grid, int nts, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_f32<kBatchTileY>, dim3(grid), dim3(NT), 0, s, films, tstart, hdr,
-                     counter, nts);
+  if (kBatchStrips)
+    hipLaunchKernelGGL(k_batch_strip_f32, dim3(grid), dim3(64), 0, s, films, tstart, hdr, counter,
+                       nts);
+  else
+    hipLaunchKernelGGL(k_batch_f32<kBatchTileY>, dim3(grid), dim3(NT), 0, s, films, tstart, hdr,
+                       counter, nts);
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, s, counter);
 }
 
 int batch_blocks_per_cu() {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_f32<kBatchTileY>, NT, 0) !=
-          hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
+  const hipError_t e =
+      kBatchStrips
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_strip_f32, 64, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_f32<kBatchTileY>, NT, 0);
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
   return per_cu;
 }
 

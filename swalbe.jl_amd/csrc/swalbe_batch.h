@@ -18,6 +18,16 @@ int film_is_small(int Lx, int Ly, int tile_y);
 
 // Rows per tile of the batched kernel.
 constexpr int kBatchTileY = 16;
+// Default rows per segment of the strip kernel; films the strip kernel
+// treats with modulo wrapping (narrower than a strip or very short).
+constexpr int kStripSeg = 64;
+constexpr int kStripOut = 58;  // output columns per strip
+int strip_is_small(int Lx, int Ly);
+// Batches step their films on strips (true) or 64 x kBatchTileY LDS tiles.
+// A film alone gives the same bits with the matching single-film variant
+// (3, or 2 with tile_y = kBatchTileY).
+constexpr bool kBatchStrips = false;
+constexpr int kBatchVariant = kBatchStrips ? 3 : 2;
 
 // One resident film of a batch (device memory, read with scalar loads).
 struct BatchFilm {
@@ -26,6 +36,7 @@ struct BatchFilm {
   float* uy[2];
   float* p;
   float* f[2];  // f[1] only when τ ≠ 1
+  // ntx, ntiles: strips and (strip, segment) items, or tile columns and tiles
   int Lx, Ly, ntx, ntiles, small, pairs, read_ft, thermal, body, pad_;
   int64_t fstride;
   uint64_t seed;

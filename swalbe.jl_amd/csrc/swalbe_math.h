@@ -166,8 +166,15 @@ SW_HD void box_muller(double u1, double u2, double* z0, double* z1) {
   *z1 = rad * s;
 }
 
-// Two standard normals for (seed, step, node, tag).  One Philox call serves
-// the node pair {2q, 2q+1}, q = node / 2: node parity b takes words 2b, 2b+1.
+// Stream index of lattice node (x, y) for the normals: 2 (x + Lx floor(y/2))
+// + y mod 2, so the vertical node pair (x, 2m), (x, 2m+1) shares one call.
+SW_HD uint32_t noise_index(int x, int y, int Lx) {
+  return 2u * ((uint32_t)x + (uint32_t)Lx * (uint32_t)(y >> 1)) + (uint32_t)(y & 1);
+}
+
+// Two standard normals for (seed, step, stream index s, tag).  One Philox
+// call serves the index pair {2q, 2q+1}, q = s / 2: parity b takes words
+// 2b, 2b+1.
 SW_HD void normal_pair(uint64_t seed, uint64_t step, uint32_t node, uint32_t tag, double* z0,
                        double* z1) {
   uint32_t r[4];

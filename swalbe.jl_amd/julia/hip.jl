@@ -44,13 +44,14 @@ struct HipStepOpts
     seed::UInt64; t0::Int64
     body_ax::Float64; body_ay::Float64; body_fac::Float64; wet_thresh::Float64
     dh_series::Ptr{Float64}; wet_series::Ptr{Int64}; mass_series::Ptr{Float64}
-    tile_y::Int32; variant::Int32; nontemporal::Int32
+    tile_y::Int32; variant::Int32; nontemporal::Int32; pad_::Int32
+    snap_every::Int64; snap_series::Ptr{Float64}
 end
 HipStepOpts(; use_theta_field = 0, thermal = 0, body = 0, materialize = 1, seed = 0, t0 = 0,
          body_ax = 0.0, body_ay = 0.0, body_fac = 0.0, wet_thresh = 0.055,
          dh_series = C_NULL, wet_series = C_NULL, mass_series = C_NULL) =
     HipStepOpts(use_theta_field, thermal, body, materialize, seed, t0, body_ax, body_ay, body_fac,
-             wet_thresh, dh_series, wet_series, mass_series, 0, 0, 0)
+             wet_thresh, dh_series, wet_series, mass_series, 0, 0, 0, 0, 0, C_NULL)
 
 function hip_check(rc::Cint, what)
     rc == HIP_OK && return nothing

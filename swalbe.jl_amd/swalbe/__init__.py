@@ -19,6 +19,7 @@ from .initialvalues import (normals, randinterface, random_film, rivulet, sinewa
                             torus, two_droplets_2d, uniforms)
 from .batch import (INIT_CAP, INIT_FLAT, INIT_NORMAL, INIT_TWO_CAPS, Batch, FilmInit, device_memory,
                     film_bytes, film_init)
+from .io import load_state, restart_from_height, save_snapshots, save_state, snapshot
 from .simulate import (run_dropletforced, run_dropletpatterned, run_dropletrelax, run_flat,
                        run_random, run_rayleightaylor)
 

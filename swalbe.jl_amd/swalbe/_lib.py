@@ -54,6 +54,7 @@ class StepOpts(C.Structure):
         ("wet_thresh", C.c_double), ("dh_series", C.POINTER(C.c_double)),
         ("wet_series", C.POINTER(C.c_int64)), ("mass_series", C.POINTER(C.c_double)),
         ("tile_y", C.c_int32), ("variant", C.c_int32), ("nontemporal", C.c_int32),
+        ("pad_", C.c_int32), ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
     ]
 
 
@@ -102,6 +103,9 @@ def lib():
     _sig(L, "swalbe_download", [_V, _I, _V, C.c_size_t])
     _sig(L, "swalbe_fill", [_V, _I, _D])
     _sig(L, "swalbe_copy", [_V, _I, _I])
+    _sig(L, "swalbe_download_raw", [_V, _I, _V, C.c_size_t])
+    _sig(L, "swalbe_upload_raw", [_V, _I, _V, C.c_size_t])
+    _sig(L, "swalbe_get_reference_height", [_V, C.POINTER(_D), C.POINTER(_D)])
     _sig(L, "swalbe_synchronize", [_V])
     _sig(L, "swalbe_equilibrium", [_V, _D])
     _sig(L, "swalbe_bgk_and_stream", [_V, _D])

@@ -464,7 +464,8 @@ def wetted(area_size, state, hthresh: float = 0.055):
 
 
 def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, thermal: bool = False,
-              seed: int = 0, materialize: bool = True, tile_y: int = 0, mass: Optional[list] = None):
+              seed: int = 0, materialize: bool = True, tile_y: int = 0, mass: Optional[list] = None,
+              snapshots: Optional[dict] = None, snapshot_every: int = 0):
     """time_loop(sys, state), time_loop(sys, state, θ), time_loop(sys, state, Δh)
     and time_loop(sys, state, f, measure) (src/simulate.jl:6-96).
 
@@ -473,7 +474,10 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     callable f(measure, state) runs the per-op kernels so that f sees the
     state between the force sum and equilibrium!, as the reference does.
     `thermal=True` subtracts the thermal! noise from F each step
-    (scripts/Rivulet_stability.jl:123-124)."""
+    (scripts/Rivulet_stability.jl:123-124).  `snapshots` (a dict) receives
+    "h_<t>" for every step t divisible by `snapshot_every`, as snapshot!
+    (src/measures.jl:99-105) would record them, copied off the device while
+    the following steps run."""
     P = sys.param
     prm = P.cparams()
     nsteps = P.Tmax
@@ -508,7 +512,15 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     if P.tdump > 0:
         ms = np.zeros(nsteps // P.tdump + 2)
         o.mass_series = ms.ctypes.data_as(C.POINTER(C.c_double))
+    snap_t = []
+    if snapshots is not None and snapshot_every > 0:
+        snap_t = [t for t in range(state.t + 1, state.t + nsteps + 1) if t % snapshot_every == 0]
+        snap_buf = np.zeros((max(1, len(snap_t)), state.Lx * state.Ly))
+        o.snap_every = int(snapshot_every)
+        o.snap_series = snap_buf.ctypes.data_as(C.POINTER(C.c_double))
     check(lib().swalbe_time_loop(state.handle, C.byref(prm), nsteps, C.byref(o)), "time_loop")
+    for k, t in enumerate(snap_t):
+        snapshots[f"h_{t}"] = snap_buf[k].reshape((state.Lx, state.Ly), order="F").copy()
     t0 = state.t
     state.t = t0 + nsteps
     if ms is not None:

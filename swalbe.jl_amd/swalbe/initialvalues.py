@@ -27,6 +27,21 @@ def normals(n: int, seed: int, step: int = 0, tag: int = TAG_INIT_NORMAL):
     return z0, z1
 
 
+def noise_index(Lx: int, Ly: int) -> np.ndarray:
+    """Stream index of every node (x, y) of an (Lx, Ly) film, Fortran order:
+    2 (x + Lx floor(y/2)) + y mod 2 (vertical node pairs share a Philox call)."""
+    x = np.arange(Lx)[:, None]
+    y = np.arange(Ly)[None, :]
+    return (2 * (x + Lx * (y // 2)) + (y % 2)).astype(np.int64)
+
+
+def lattice_normals(Lx: int, Ly: int, seed: int, step: int = 0, tag: int = TAG_INIT_NORMAL):
+    """The two normals of every node of an (Lx, Ly) film, shape (Lx, Ly) each."""
+    idx = noise_index(Lx, Ly)
+    z0, z1 = normals(int(idx.max()) + 1, seed, step, tag)
+    return z0[idx], z1[idx]
+
+
 def uniforms(n: int, seed: int, step: int = 0, tag: int = TAG_INIT_UNIFORM):
     u = np.empty(n)
     check(lib().swalbe_host_uniforms(int(seed), int(step), int(tag), int(n),
@@ -37,16 +52,16 @@ def uniforms(n: int, seed: int, step: int = 0, tag: int = TAG_INIT_UNIFORM):
 def randinterface(height: np.ndarray, h0: float, eps: float, seed: int = 0) -> None:
     """randinterface!(height, h₀, ϵ) (src/initialvalues.jl:23-33):
     height .= h₀ (1 + ϵ N(0,1)), normals from the counter stream."""
-    z0, _ = normals(height.size, seed)
-    height[...] = h0 * (1.0 + eps * z0.reshape(height.shape, order="F"))
+    z0, _ = lattice_normals(height.shape[0], height.shape[1], seed)
+    height[...] = h0 * (1.0 + eps * z0)
 
 
 def random_film(Lx: int, Ly: int, seed: int, base: float, amp: float, dist: str = "normal"):
     """h = base + amp * N(0,1) (or U(0,1)) per node, Fortran order."""
     if dist == "normal":
-        z, _ = normals(Lx * Ly, seed)
-    else:
-        z = uniforms(Lx * Ly, seed)
+        z, _ = lattice_normals(Lx, Ly, seed)
+        return base + amp * z
+    z = uniforms(Lx * Ly, seed)
     return (base + amp * z).reshape((Lx, Ly), order="F")
 
 

@@ -27,6 +27,10 @@ import numpy as np
 
 from ._lib import ENOMEM, FilmInit, SwalbeError
 
+# single-film kernel variant and tile height whose arithmetic the batched
+# kernel shares (LDS tiles of 64 x 16, collide at the destination)
+BATCH_VARIANT, BATCH_TILE_Y = 2, 16
+
 MASK64 = (1 << 64) - 1
 
 
@@ -389,7 +393,9 @@ def solve_alone(r: Request, mode: str = "fused", T=np.float32) -> Dict[str, np.n
         st.copy("ftemp", "feq")
     if mode == "fused":
         o = _lib.StepOpts()
-        o.thermal, o.seed, o.tile_y = int(r.init.thermal), int(r.init.seed), 16
+        # the kernel variant batches use, so that the bits agree
+        o.thermal, o.seed = int(r.init.thermal), int(r.init.seed)
+        o.variant, o.tile_y = BATCH_VARIANT, BATCH_TILE_Y
         o.body = int(r.init.body)
         o.body_ax, o.body_ay, o.body_fac = r.init.body_ax, r.init.body_ay, r.init.body_fac
         prm = r.param.cparams()

@@ -242,6 +242,16 @@ FP32_CASES = [
     ((192, 64), 8, 2, "tau"),
     ((192, 64), 16, 1, "body"),
     ((192, 64), 16, 2, "body"),
+    # LDS-free strips: tile_y = rows per segment
+    ((256, 96), 64, 3, "thermal"),
+    ((256, 96), 16, 3, "thermal"),
+    ((200, 130), 48, 3, "thermal"),     # Ly not a multiple of the segment
+    ((199, 61), 0, 3, "thermal"),       # odd Lx, odd Ly
+    ((40, 20), 0, 3, "thermal"),        # narrower than a strip: modulo wrap
+    ((130, 7), 4, 3, "thermal"),        # fewer rows than the halo
+    ((192, 64), 32, 3, "theta"),
+    ((192, 64), 64, 3, "tau"),
+    ((192, 64), 5, 3, "body"),
 ]
 
 
@@ -294,16 +304,20 @@ def test_fp32_deviation_mode_tolerance(S, shape, tile, variant, feat):
     assert abs(m - ost.height.sum()) <= 1e-6 * ost.height.sum()
 
 
-def test_fp32_noise_matches_fp64_normals(S):
-    """Strong noise (kbt = 1e-3) dominates the forcing, so the FP32 kernel's
-    hardware Box-Muller must draw the FP64 normals to FP32 precision."""
-    Lx, Ly, nsteps = 128, 64, 20
+@pytest.mark.parametrize("shape", [(128, 64), (130, 63), (70, 9)], ids=str)
+def test_fp32_noise_matches_fp64_normals(S, shape):
+    """Strong noise (kbt = 1e-3) dominates the forcing, so the FP32 kernels'
+    hardware Box-Muller must draw the FP64 normals to FP32 precision: vertical
+    pairs sharing a Philox call, odd Ly (an unpaired last row) and short
+    films whose rows wrap inside a strip segment."""
+    Lx, Ly = shape
+    nsteps = 20
     h0 = np.ones((Lx, Ly), order="F")
     prm = O.params(kbt=1e-3, tdump=100)
     ost = _oracle_state(h0)
     O.time_loop(ost, prm, nsteps, thermal=True, seed=9)
     sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=100, kbt=1e-3))
-    for variant in (1, 2):
+    for variant in (1, 2, 3):
         st = S.Sys(sysc, "GPU", T=np.float32)
         st.height = h0
         S.equilibrium(st, sysc)
@@ -341,3 +355,34 @@ def test_plane_padding_keeps_fp64_bitexact(S, monkeypatch):
     st32 = S.Sys(sysc, "GPU", T=np.float32)
     st32.feq = f
     assert np.allclose(st32.get("feq", np.float64), f, atol=1e-6)
+
+
+@pytest.mark.parametrize("variant", [2, 3])
+def test_fp32_series_match_oracle(S, variant):
+    """On-device Δh and wetted series of the FP32 kernels (the per-step
+    max - min and count(h > 0.055) of the input height) against the oracle."""
+    Lx, Ly, nsteps = 150, 70, 60
+    h0 = O.init_normal(Lx, Ly, 8, 0.06, 0.01)
+    prm = O.params(kbt=1e-6, tdump=1000)
+    ost = _oracle_state(h0)
+    ser = O.time_loop(ost, prm, nsteps, thermal=True, seed=2, dh=True, wetted=0.055)
+    import ctypes as C
+    from swalbe import _lib
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=1000, kbt=1e-6))
+    st = S.Sys(sysc, "GPU", T=np.float32)
+    st.set_reference_height(0.06)
+    st.height = h0
+    S.equilibrium(st, sysc)
+    dh = np.zeros(nsteps)
+    wet = np.zeros(nsteps, dtype=np.int64)
+    o = _lib.StepOpts()
+    o.thermal, o.seed, o.variant, o.wet_thresh = 1, 2, variant, 0.055
+    o.dh_series = dh.ctypes.data_as(C.POINTER(C.c_double))
+    o.wet_series = wet.ctypes.data_as(C.POINTER(C.c_int64))
+    _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(sysc.param.cparams()), nsteps,
+                                           C.byref(o)))
+    ref_dh = np.asarray(ser["dh"])
+    ref_wet = np.asarray(ser["wetted"])
+    assert np.all(np.abs(dh - ref_dh) <= 1e-7 + 1e-4 * ref_dh), np.max(np.abs(dh - ref_dh))
+    # counts may differ only where h sits within FP32 rounding of the threshold
+    assert np.max(np.abs(wet - ref_wet)) <= 3, np.max(np.abs(wet - ref_wet))

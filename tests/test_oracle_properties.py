@@ -249,3 +249,27 @@ def test_random_interface_flattens():
     st = _random_state(25, 25, 5, amp=0.1)
     O.time_loop(st, O.params(tdump=5000), 10000, theta_field=np.full((25, 25), 1 / 9))
     assert st.height.max() - st.height.min() < 0.1
+
+
+def test_noise_index_pairs_vertical_neighbours():
+    """Stream indices of the normals: a bijection of the lattice into
+    [0, 2 Lx ceil(Ly/2)), with (x, 2m) and (x, 2m+1) sharing one Philox call
+    (index // 2) and taking its word pairs 0-1 and 2-3 (index % 2)."""
+    for Lx, Ly in ((7, 5), (6, 4), (1, 3)):
+        idx = np.array([[O.noise_index(x, y, Lx) for y in range(Ly)] for x in range(Lx)])
+        assert len(set(idx.ravel().tolist())) == Lx * Ly
+        assert idx.max() < 2 * Lx * ((Ly + 1) // 2)
+        for x in range(Lx):
+            for y in range(0, Ly - 1, 2):
+                assert idx[x, y] // 2 == idx[x, y + 1] // 2
+                assert idx[x, y] % 2 == 0 and idx[x, y + 1] % 2 == 1
+    # the thermal noise of node (x, y) is the normal pair of its stream index
+    Lx, Ly = 5, 4
+    h = np.ones((Lx, Ly), order="F")
+    kx, ky = O.thermal(h, kbt=1.0, mu=1 / 6, delta=1.0, seed=3, step=2)
+    amp = np.sqrt(2 * 1.0 * (1 / 6) * 6 / (2 + 6 + 3))
+    for x in range(Lx):
+        for y in range(Ly):
+            z0, z1 = O.normal_pair(3, 2, O.noise_index(x, y, Lx), 0)
+            assert kx[x, y] == pytest.approx(z0 * amp, rel=1e-14)
+            assert ky[x, y] == pytest.approx(z1 * amp, rel=1e-14)
