@@ -373,6 +373,37 @@ def latency_stats(reqs: Sequence[Request]) -> dict:
             "mean_s": float(lat.mean()), "glups": lups / span / 1e9}
 
 
+def perop_step_seconds(r: Request, steps: int = 20, warm: int = 3) -> float:
+    """Seconds per step of the op-by-op path on r's film (state set up
+    beforehand, device synchronised around the timed steps)."""
+    from .core import (BGKandStream, Sys, SysConst, equilibrium, filmpressure, force_sum,
+                       h_grad_p, moments, slippage)
+    sysc = SysConst(r.Lx, r.Ly, r.param)
+    st = Sys(sysc, "GPU", T=np.float32)
+    st.set_reference_height(r.init.href, r.param.g)
+    st.init_height(r.init)
+    equilibrium(st, sysc)
+
+    def one():
+        filmpressure(st, sysc)
+        h_grad_p(st)
+        slippage(st, sysc)
+        force_sum(st, False)
+        equilibrium(st, sysc)
+        BGKandStream(st, sysc)
+        moments(st)
+    for _ in range(warm):
+        one()
+    st.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        one()
+    st.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    st.close()
+    return dt
+
+
 def solve_alone(r: Request, mode: str = "fused", T=np.float32) -> Dict[str, np.ndarray]:
     """The request run by itself on the GPU: a single-film state, the same
     initial height, then `mode` = "fused" (one fused kernel per step) or
@@ -420,5 +451,6 @@ def solve_alone(r: Request, mode: str = "fused", T=np.float32) -> Dict[str, np.n
 
 
 __all__ = ["Request", "Server", "NativeServer", "BatchBackend", "coalescence_sweep", "poisson_requests",
+           "perop_step_seconds",
            "coalescence_request", "two_caps_film", "solve_alone", "latency_stats", "domain_seed",
            "splitmix64"]
