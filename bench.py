@@ -33,7 +33,7 @@ WORKLOADS = {
     "droplet1024_f32": (1024, 1024, "f32", False),
 }
 # many-film workloads (FP32 batched kernel) and the copy calibration
-BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson", "copy4096_f32")
+BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson", "copy4096_f32", "pattern4096_f32")
 
 
 def bytes_per_lup(esz: int, tau_is_one: bool) -> int:
@@ -135,6 +135,28 @@ def bench_copy(a):
                       "dtype": "f32", "data": "synthetic",
                       "config": {"workload": "copy4096_f32: 9 planes of 4096^2 FP32",
                                  "bytes_per_copy": nbytes}}), flush=True)
+
+
+def bench_pattern(a):
+    """The fused step's data movement alone at 4096^2 FP32: read 3 planes (h,
+    ux, uy), write 13 (9 f, h, ux, uy, p) over the step's 64 x 16 tiles, no
+    arithmetic: the ceiling the step's access pattern allows."""
+    import ctypes as C
+    from swalbe import _lib
+    ms = C.c_double()
+    L = 4096
+    ty = a.tile_y or 16
+    _lib.check(_lib.lib().swalbe_pattern_probe(L, L, 3, 13, ty, a.steps, C.byref(ms)), "probe")
+    t = ms.value / 1e3
+    gbs = 64 * L * L / t / 1e9
+    print(json.dumps({"metric": "data-movement ceiling of the fused step's access pattern",
+                      "value": round(L * L / t / 1e9, 3), "unit": "GLUPS", "n_gpus": 1,
+                      "steps": a.steps, "warmup": 3, "ms_per_step": round(ms.value, 5),
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "pattern4096_f32: read 3 / write 13 planes of "
+                                             f"4096^2 FP32 over 64x{ty} tiles, no arithmetic"},
+                      "achieved_GBs": round(gbs, 1)}), flush=True)
 
 
 def bench_serve(a):
@@ -240,6 +262,8 @@ def main():
         return bench_serve(a)
     if a.workload == "copy4096_f32":
         return bench_copy(a)
+    if a.workload == "pattern4096_f32":
+        return bench_pattern(a)
 
     import ctypes as C
     import numpy as np

@@ -262,6 +262,13 @@ int swalbe_init_height(swalbe_state *st, const swalbe_film_init *init);
 /* free and total device memory (admission by HBM footprint) */
 int swalbe_device_memory(size_t *free_bytes, size_t *total_bytes);
 
+/* ---- measurement: the data-movement ceiling of the fused step's access
+ * pattern (no reference counterpart).  Reads nread planes and writes nwrite
+ * planes of an Lx x Ly FP32 film over 64 x tile_y tiles with no arithmetic;
+ * returns the mean device time per launch over `iters` launches. */
+int swalbe_pattern_probe(int32_t Lx, int32_t Ly, int32_t nread, int32_t nwrite, int32_t tile_y,
+                         int32_t iters, double *ms_per_launch);
+
 /* ---- raw-array twins on caller-owned device arrays (plain values of the
  * given dtype, Julia layout), enqueued on `stream` (NULL = default stream).
  * These replace the positional-array methods the reference exposes. */

@@ -156,6 +156,7 @@ def lib():
     _sig(L, "swalbe_slippage_ptr", [_I, _V, _V, _V, _V, _V, _D, _D, _I, _I, _V])
     _sig(L, "swalbe_thermal_ptr", [_I, _V, _V, _V, _D, _D, _D, C.c_uint64, C.c_uint64, _I, _I, _V])
     _sig(L, "swalbe_inclination_ptr", [_I, _V, _V, _V, _D, _D, _D, _I, _I, _V])
+    _sig(L, "swalbe_pattern_probe", [_I, _I, _I, _I, _I, _I, C.POINTER(_D)])
     _sig(L, "swalbe_malloc", [C.POINTER(_V), C.c_size_t])
     _sig(L, "swalbe_free", [_V])
     _sig(L, "swalbe_memcpy_h2d", [_V, _V, C.c_size_t])
