@@ -104,6 +104,10 @@ int swalbe_copy(swalbe_state *st, int32_t dst_field, int32_t src_field);
 /* state.<field> .= value (src/simulate.jl:241, test/collide.jl:19-22). */
 int swalbe_fill(swalbe_state *st, int32_t field, double value);
 int swalbe_synchronize(swalbe_state *st);
+/* Enqueue the state's work on a caller's HIP stream (a hipStream_t; NULL
+ * restores the state's own stream).  Work already enqueued completes first.
+ * The caller keeps the stream alive while the state uses it. */
+int swalbe_set_stream(swalbe_state *st, void *stream);
 
 /* ---- one op per reference function, on a state ------------------------ */
 /* equilibrium!(state, sys) -- src/equilibrium.jl:63-128 */

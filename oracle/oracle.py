@@ -368,3 +368,184 @@ def time_loop(state: State, prm: Params, nsteps: int, *, theta_field=None, therm
     rc = lib().orc_time_loop(C.byref(s), C.byref(prm), int(nsteps), C.byref(o))
     assert rc == 0
     return out
+
+
+# ---------------------------------------------------------------- one dimension (D1Q3)
+FIELDS_1D = ("fout", "ftemp", "feq", "height", "vel", "pressure", "F", "slip", "hgp", "kbt")
+
+
+class State1DC(C.Structure):
+    _fields_ = [("L", C.c_int32)] + [(name, _D) for name in FIELDS_1D]
+
+
+class LoopOpts1D(C.Structure):
+    _fields_ = [
+        ("theta_field", _D), ("thermal", C.c_int32), ("body", C.c_int32),
+        ("alpha", C.c_double), ("body_fac", C.c_double), ("border1", _D), ("border2", _D),
+        ("seed", C.c_uint64), ("t0", C.c_int64), ("dh_series", _D), ("wet_series", _I64),
+        ("wet_thresh", C.c_double), ("mass_series", _D),
+    ]
+
+
+class State1D:
+    """State_1D{Float64} (src/initialize.jl:276-290) as numpy arrays: zeros
+    except height = 1, as Sys(sysc::Consts_1D) allocates them (:587-600)."""
+
+    def __init__(self, L):
+        self.L = L
+        for name in FIELDS_1D:
+            shape = (L, 3) if name in ("fout", "ftemp", "feq") else (L,)
+            setattr(self, name, np.zeros(shape, order="F"))
+        self.height[...] = 1.0
+
+    def cstruct(self):
+        s = State1DC()
+        s.L = self.L
+        for name in FIELDS_1D:
+            arr = getattr(self, name)
+            assert arr.flags.f_contiguous and arr.dtype == np.float64
+            setattr(s, name, _p(arr))
+        return s
+
+
+def equilibrium_1d(h, u, g):
+    h = _f(h)
+    L = h.shape[0]
+    feq = np.zeros((L, 3), order="F")
+    lib().orc1d_equilibrium(_p(feq), _p(h), _p(_f(u, (L,))), C.c_double(g), L)
+    return feq
+
+
+def bgk_and_stream_1d(feq, ftemp, F, tau, border1=None, border2=None):
+    feq = _f(feq)
+    L = feq.shape[0]
+    ft = _f(ftemp, (L, 3))
+    fout = np.zeros((L, 3), order="F")
+    b1 = _f(border1, (L,)) if border1 is not None else None
+    b2 = _f(border2, (L,)) if border2 is not None else None
+    lib().orc1d_bgk_and_stream(_p(fout), _p(feq), _p(ft), _p(_f(F, (L,))), C.c_double(tau),
+                               _p(b1), _p(b2), L)
+    return fout, ft
+
+
+def moments_1d(f):
+    f = _f(f)
+    L = f.shape[0]
+    h, u = np.zeros(L), np.zeros(L)
+    lib().orc1d_moments(_p(h), _p(u), _p(f), L)
+    return h, u
+
+
+def filmpressure_1d(h, gamma, theta, n, m, hmin, hcrit, theta_field=None):
+    h = _f(h)
+    L = h.shape[0]
+    p = np.zeros(L)
+    tf = _f(theta_field, (L,)) if theta_field is not None else None
+    lib().orc1d_filmpressure(_p(p), _p(h), C.c_double(gamma), C.c_double(theta), _p(tf), n, m,
+                             C.c_double(hmin), C.c_double(hcrit), L)
+    return p
+
+
+def filmpressure_raw_1d(h, gamma, theta, n, m, hmin, hcrit):
+    h = _f(h)
+    L = h.shape[0]
+    p = np.zeros(L)
+    rc = lib().orc1d_filmpressure_raw(_p(p), _p(h), C.c_double(gamma), C.c_double(theta), n, m,
+                                      C.c_double(hmin), C.c_double(hcrit), L)
+    if rc != 0:
+        raise ValueError("DomainError: (n, m) must be (9, 3) or (3, 2)")
+    return p
+
+
+def grad_1d(f, a=None):
+    f = _f(f)
+    L = f.shape[0]
+    out = np.zeros(L)
+    lib().orc1d_grad(_p(out), _p(f), _p(_f(a, (L,)) if a is not None else None), L)
+    return out
+
+
+def laplacian_1d(f):
+    f = _f(f)
+    out = np.zeros(f.shape[0])
+    lib().orc1d_laplacian(_p(out), _p(f), f.shape[0])
+    return out
+
+
+def hgradp_1d(p, h):
+    h = _f(h)
+    L = h.shape[0]
+    out = np.zeros(L)
+    lib().orc1d_hgradp(_p(out), _p(_f(p, (L,))), _p(h), L)
+    return out
+
+
+def slippage_1d(h, u, delta, mu):
+    h = _f(h)
+    L = h.shape[0]
+    out = np.zeros(L)
+    lib().orc1d_slippage(_p(out), _p(h), _p(_f(u, (L,))), C.c_double(delta), C.c_double(mu), L)
+    return out
+
+
+def thermal_1d(h, kbt, mu, delta, seed, step):
+    h = _f(h)
+    L = h.shape[0]
+    out = np.zeros(L)
+    lib().orc1d_thermal(_p(out), _p(h), C.c_double(kbt), C.c_double(mu), C.c_double(delta),
+                        C.c_uint64(seed), C.c_uint64(step), L)
+    return out
+
+
+def inclination_1d(F, h, alpha, fac):
+    F = _f(F)
+    lib().orc1d_inclination(_p(F), _p(_f(h, F.shape)), C.c_double(alpha), C.c_double(fac),
+                            F.shape[0])
+    return F
+
+
+def obslist_1d(obs):
+    """obslist1D(obs) (src/obstacle.jl:6-35): interior and the border masks
+    [obsright, obsleft] of an obstacle vector, periodic neighbours."""
+    obs = np.asarray(obs, dtype=float)
+    right, left = np.roll(obs, -1), np.roll(obs, 1)  # obs[i+1], obs[i-1]
+    interior = ((obs == 1) & (right == 1) & (left == 1)).astype(float)
+    obsleft = ((obs == 1) & (left == 1)).astype(float)
+    obsright = ((obs == 1) & (right == 1)).astype(float)
+    return interior, (obsright, obsleft)
+
+
+def time_loop_1d(state: State1D, prm: Params, nsteps: int, *, theta_field=None, thermal=False,
+                 seed=0, t0=0, body=None, borders=None, dh=False, wetted=None, mass=False):
+    """The 1D time loop; body = (α, fac); borders = (border1, border2)."""
+    o = LoopOpts1D()
+    keep = []
+    if theta_field is not None:
+        tf = _f(theta_field, (state.L,))
+        keep.append(tf)
+        o.theta_field = _p(tf)
+    o.thermal, o.seed, o.t0 = int(thermal), seed, t0
+    if body is not None:
+        o.body, o.alpha, o.body_fac = 1, body[0], body[1]
+    if borders is not None:
+        b1, b2 = _f(borders[0], (state.L,)), _f(borders[1], (state.L,))
+        keep += [b1, b2]
+        o.border1, o.border2 = _p(b1), _p(b2)
+    out = {}
+    if dh:
+        out["dh"] = np.zeros(nsteps)
+        o.dh_series = _p(out["dh"])
+    if wetted is not None:
+        out["wetted"] = np.zeros(nsteps, dtype=np.int64)
+        o.wet_series = out["wetted"].ctypes.data_as(_I64)
+        o.wet_thresh = wetted
+    if mass:
+        out["mass"] = np.zeros(max(1, nsteps // max(1, prm.tdump) + 1))
+        o.mass_series = _p(out["mass"])
+    L = lib()
+    L.orc1d_time_loop.restype = C.c_int
+    L.orc1d_time_loop.argtypes = [C.POINTER(State1DC), C.POINTER(Params), C.c_int64,
+                                  C.POINTER(LoopOpts1D)]
+    cs = state.cstruct()
+    L.orc1d_time_loop(C.byref(cs), C.byref(prm), nsteps, C.byref(o))
+    return out

@@ -104,6 +104,49 @@ void orc_init_normal(double *h, int Lx, int Ly, uint64_t seed, double base, doub
 /* ---- driver ---- */
 int orc_time_loop(orc_state *st, const orc_params *p, int64_t nsteps, const orc_loop_opts *o);
 
+/* ---- one dimension: D1Q3 (State_1D, src/initialize.jl:276-290); fields of L
+ * nodes, distributions (L, 3), population k at offset k*L ---- */
+typedef struct orc1d_state {
+    int32_t L;
+    double *fout, *ftemp, *feq;
+    double *height, *vel, *pressure, *F, *slip, *hgp, *kbt;
+} orc1d_state;
+
+typedef struct orc1d_loop_opts {
+    const double *theta_field; /* time_loop(sys, state, θ) with a θ vector */
+    int32_t thermal;           /* F -= thermal! noise each step */
+    int32_t body;              /* inclination!(α, state) callback */
+    double alpha, body_fac;
+    const double *border1, *border2; /* bounce-back masks (SysConstWithBound_1D), or NULL */
+    uint64_t seed;
+    int64_t t0;
+    double *dh_series;
+    int64_t *wet_series;
+    double wet_thresh;
+    double *mass_series;
+} orc1d_loop_opts;
+
+void orc1d_equilibrium(double *feq, const double *h, const double *u, double g, int L);
+void orc1d_bgk_and_stream(double *fout, const double *feq, double *ftemp, const double *F,
+                          double tau, const double *border1, const double *border2, int L);
+void orc1d_moments(double *h, double *u, const double *f, int L);
+void orc1d_filmpressure(double *p, const double *h, double gamma, double theta,
+                        const double *theta_field, int n, int m, double hmin, double hcrit,
+                        int L);
+int orc1d_filmpressure_raw(double *p, const double *h, double gamma, double theta, int n, int m,
+                           double hmin, double hcrit, int L);
+void orc1d_grad(double *out, const double *f, const double *a, int L);
+void orc1d_laplacian(double *out, const double *f, int L);
+void orc1d_hgradp(double *hgp, const double *p, const double *h, int L);
+void orc1d_slippage(double *slip, const double *h, const double *u, double delta, double mu,
+                    int L);
+void orc1d_thermal(double *fluc, const double *h, double kbt, double mu, double delta,
+                   uint64_t seed, uint64_t step, int L);
+void orc1d_inclination(double *F, const double *h, double alpha, double fac, int L);
+void orc1d_force_sum(double *F, const double *hgp, const double *slip, const double *kbt, int L);
+int orc1d_time_loop(orc1d_state *st, const orc_params *p, int64_t nsteps,
+                    const orc1d_loop_opts *o);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@ struct swalbe_state {
   swalbe::Film film;
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;  // the stream created with the state
   double* partials = nullptr;  // reduction scratch, 4 doubles per block
   int nparts = 0;
   double* red_out = nullptr;   // 4 doubles
@@ -264,6 +265,7 @@ int swalbe_create(int32_t Lx, int32_t Ly, int32_t dtype, int32_t kind, swalbe_st
   if (rc == SWALBE_OK) {
     hipError_t e = hipStreamCreate(&st->stream);
     if (e != hipSuccess) rc = hip_fail(e, "hipStreamCreate");
+    st->own_stream = st->stream;
   }
   if (rc != SWALBE_OK) {
     swalbe_destroy(st);
@@ -299,6 +301,7 @@ int swalbe_destroy(swalbe_state* st) {
   if (!st) return SWALBE_OK;
   hipSetDevice(st->device);
   if (st->stream) hipStreamSynchronize(st->stream);
+  if (st->own_stream && st->own_stream != st->stream) hipStreamSynchronize(st->own_stream);
   Film& F = st->film;
   void* ptrs[] = {F.f[0],  F.f[1],  F.feq,   F.hbuf[0], F.hbuf[1], F.uxbuf[0], F.uxbuf[1],
                   F.uybuf[0], F.uybuf[1], F.vsq, F.pressure, F.Fx, F.Fy, F.slipx, F.slipy,
@@ -306,8 +309,17 @@ int swalbe_destroy(swalbe_state* st) {
                   st->red_out};
   for (void* p : ptrs)
     if (p) hipFree(p);
-  if (st->stream) hipStreamDestroy(st->stream);
+  if (st->own_stream) hipStreamDestroy(st->own_stream);
   delete st;
+  return SWALBE_OK;
+}
+
+int swalbe_set_stream(swalbe_state* st, void* stream) {
+  SW_CHECK_STATE(st);
+  if (int rc = set_device(st)) return rc;
+  // work already enqueued on the old stream finishes first
+  SW_HIP(hipStreamSynchronize(st->stream));
+  st->stream = stream ? (hipStream_t)stream : st->own_stream;
   return SWALBE_OK;
 }
 

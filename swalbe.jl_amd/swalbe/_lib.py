@@ -107,6 +107,7 @@ def lib():
     _sig(L, "swalbe_upload_raw", [_V, _I, _V, C.c_size_t])
     _sig(L, "swalbe_get_reference_height", [_V, C.POINTER(_D), C.POINTER(_D)])
     _sig(L, "swalbe_synchronize", [_V])
+    _sig(L, "swalbe_set_stream", [_V, _V])
     _sig(L, "swalbe_equilibrium", [_V, _D])
     _sig(L, "swalbe_bgk_and_stream", [_V, _D])
     _sig(L, "swalbe_moments", [_V])

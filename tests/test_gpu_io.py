@@ -55,3 +55,27 @@ def test_restart_from_saved_state_is_bit_exact(S, T, tau, tmp_path):
     S.time_loop(sysc, b, thermal=True, seed=8)
     for name in ("height", "velx", "vely", "pressure", "fout"):
         assert np.array_equal(a.get(name, np.float64), b.get(name, np.float64)), name
+
+
+def test_state_on_a_caller_stream(S):
+    """swalbe_set_stream: the same loop on a caller's HIP stream gives the same
+    bits, and NULL restores the state's own stream."""
+    import ctypes as C
+    from swalbe import _lib
+    hip = C.CDLL("libamdhip64.so")
+    stream = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(stream)) == 0
+    try:
+        sysc, a = _state(S, np.float32, 1.0)
+        sysc2, b = _state(S, np.float32, 1.0)
+        sysc.param.Tmax = sysc2.param.Tmax = 30
+        _lib.check(_lib.lib().swalbe_set_stream(b.handle, stream), "set_stream")
+        S.time_loop(sysc, a, thermal=True, seed=4)
+        S.time_loop(sysc2, b, thermal=True, seed=4)
+        _lib.check(_lib.lib().swalbe_set_stream(b.handle, None), "set_stream")
+        S.time_loop(sysc, a, thermal=True, seed=4)
+        S.time_loop(sysc2, b, thermal=True, seed=4)
+        assert np.array_equal(a.get("height"), b.get("height"))
+        b.close()
+    finally:
+        assert hip.hipStreamDestroy(stream) == 0

@@ -1,0 +1,120 @@
+"""The 1D (D1Q3) oracle against the reference's known answers and against
+what the mathematics fixes: equilibrium moments, mass conservation (with
+and without bounce-back walls), streaming as a roll, the noise amplitude."""
+import numpy as np
+import pytest
+
+import oracle as O
+from kat1d import load_cases, run_case
+
+
+class Oracle1D:
+    equilibrium = staticmethod(O.equilibrium_1d)
+
+    @staticmethod
+    def bgk_and_stream(feq, ftemp, F, tau):
+        return O.bgk_and_stream_1d(feq, ftemp, F, tau)[0]
+
+    moments = staticmethod(O.moments_1d)
+
+    @staticmethod
+    def filmpressure(h, gamma, theta, n, m, hmin, hcrit, form):
+        if form == "raw":
+            return O.filmpressure_raw_1d(h, gamma, theta, n, m, hmin, hcrit)
+        return O.filmpressure_1d(h, gamma, theta, n, m, hmin, hcrit)
+
+    grad = staticmethod(O.grad_1d)
+    laplacian = staticmethod(O.laplacian_1d)
+    slippage = staticmethod(O.slippage_1d)
+
+    @staticmethod
+    def hgradp(p, h):
+        return O.hgradp_1d(p, h)
+
+    thermal = staticmethod(O.thermal_1d)
+    inclination = staticmethod(O.inclination_1d)
+
+    @staticmethod
+    def run_flat(L, Tmax, tdump):
+        st = O.State1D(L)
+        st.height[...] = 1.0
+        O.time_loop_1d(st, O.params(tdump=tdump), Tmax)
+        return st.height
+
+
+CASES = load_cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_oracle_1d_reference_kat(case):
+    run_case(Oracle1D, case)
+
+
+def test_equilibrium_moments_closed_form():
+    rng = np.random.default_rng(1)
+    h, u = rng.uniform(0.5, 2.0, 40), rng.uniform(-0.1, 0.1, 40)
+    for g in (0.0, 0.3):
+        f = O.equilibrium_1d(h, u, g)
+        assert np.allclose(f.sum(axis=1), h, rtol=1e-14)
+        assert np.allclose(f[:, 1] - f[:, 2], h * u, rtol=1e-13, atol=1e-16)
+        # second moment: h u^2 + g h^2 / 2 (shallow water)
+        assert np.allclose(f[:, 1] + f[:, 2], h * u * u + 0.5 * g * h * h, rtol=1e-12)
+
+
+def test_streaming_is_a_roll_and_force_moment():
+    rng = np.random.default_rng(2)
+    L = 17
+    feq = rng.uniform(0.1, 1.0, (L, 3))
+    F = rng.uniform(-0.01, 0.01, L)
+    fout, _ = O.bgk_and_stream_1d(feq, np.zeros((L, 3)), F, 1.0)
+    assert np.array_equal(fout[:, 0], feq[:, 0])
+    assert np.array_equal(fout[:, 1], np.roll(feq[:, 1] + 0.5 * F, 1))
+    assert np.array_equal(fout[:, 2], np.roll(feq[:, 2] - 0.5 * F, -1))
+
+
+def test_mass_is_conserved_with_and_without_walls():
+    L = 120
+    st = O.State1D(L)
+    x = np.arange(L)
+    st.height[...] = 1.0 + 0.05 * np.sin(2 * np.pi * 3 * x / L)
+    st.feq[...] = O.equilibrium_1d(st.height, st.vel, 0.0)
+    m0 = st.height.sum()
+    O.time_loop_1d(st, O.params(tdump=10 ** 9), 500)
+    assert abs(st.height.sum() - m0) <= 1e-12 * m0
+    # bounce-back walls at both ends of the film (four-node obstacles)
+    obs = np.zeros(L)
+    obs[:4] = obs[-4:] = 1
+    _, (b1, b2) = O.obslist_1d(obs)
+    st2 = O.State1D(L)
+    st2.height[...] = 1.0 + 0.05 * np.sin(2 * np.pi * x / L)
+    st2.feq[...] = O.equilibrium_1d(st2.height, st2.vel, 0.0)
+    m0 = st2.height.sum()
+    O.time_loop_1d(st2, O.params(tdump=10 ** 9), 300, borders=(b1, b2))
+    assert abs(st2.height.sum() - m0) <= 1e-10 * m0
+    # the walls change the flow: the same film without them differs
+    st3 = O.State1D(L)
+    st3.height[...] = 1.0 + 0.05 * np.sin(2 * np.pi * x / L)
+    st3.feq[...] = O.equilibrium_1d(st3.height, st3.vel, 0.0)
+    O.time_loop_1d(st3, O.params(tdump=10 ** 9), 300)
+    assert np.max(np.abs(st3.height - st2.height)) > 1e-6
+
+
+def test_obslist_matches_the_reference_example():
+    # test/collide.jl:131-138: walls at the first and last node
+    obs = np.zeros(30)
+    obs[0] = obs[-1] = 1
+    interior, (right, left) = O.obslist_1d(obs)
+    assert interior.sum() == 0
+    assert right[-1] == 1 and right.sum() == 1   # obs[i] and obs[i+1] (periodic)
+    assert left[0] == 1 and left.sum() == 1      # obs[i] and obs[i-1]
+
+
+def test_rayleigh_taylor_1d_grows():
+    # test/simulate.jl:38-40: L = 512, g = -0.002, k = 4, ε = 0.01
+    L = 512
+    st = O.State1D(L)
+    i = np.arange(1, L + 1)
+    st.height[...] = 1.0 * (1 + 0.01 * np.sin(2 * np.pi * 4 * i / (L - 1)))
+    st.feq[...] = O.equilibrium_1d(st.height, st.vel, -0.002)
+    out = O.time_loop_1d(st, O.params(g=-0.002, tdump=500), 1000, dh=True)
+    assert out["dh"][0] < out["dh"][-1]
