@@ -273,6 +273,132 @@ int swalbe_device_memory(size_t *free_bytes, size_t *total_bytes);
 int swalbe_pattern_probe(int32_t Lx, int32_t Ly, int32_t nread, int32_t nwrite, int32_t tile_y,
                          int32_t iters, double *ms_per_launch);
 
+/* ======================================================================
+ * One-dimensional films (D1Q3, velocities c = 0, +1, -1).
+ * ====================================================================== */
+
+/* Fields of a 1D film: the arrays of State_1D (src/initialize.jl:276-290),
+ * kbt of State_thermal_1D (:338-341), γ and ∇γ of State_gamma_1D (:304-308),
+ * θ as time_loop(sys, state, θ) takes it (src/simulate.jl:118), the wall
+ * masks of SysConstWithBound_1D (:112-120; border1 = obsright, border2 =
+ * obsleft of obslist1D, src/obstacle.jl:6-35) and its interior mask.
+ * Distributions are (L, 3) column-major: population k at offset k*L. */
+enum swalbe1d_field {
+    SWALBE1D_FOUT = 0,
+    SWALBE1D_FTEMP = 1,
+    SWALBE1D_FEQ = 2,
+    SWALBE1D_HEIGHT = 3,
+    SWALBE1D_VEL = 4,
+    SWALBE1D_PRESSURE = 5,
+    SWALBE1D_F = 6,
+    SWALBE1D_SLIP = 7,
+    SWALBE1D_HGRADP = 8,
+    SWALBE1D_KBT = 9,
+    SWALBE1D_THETA = 10,     /* optional fields: allocated on first upload / fill */
+    SWALBE1D_GAMMA = 11,
+    SWALBE1D_GRADGAMMA = 12,
+    SWALBE1D_BORDER1 = 13,
+    SWALBE1D_BORDER2 = 14,
+    SWALBE1D_INTERIOR = 15,
+    SWALBE1D_NFIELDS = 16
+};
+
+typedef struct swalbe_film1d swalbe_film1d;
+
+/* Sys(sysc::Consts_1D; T, kind) (src/initialize.jl:587-616): all fields zero
+ * except height = 1.  kind SWALBE_SIMPLE or SWALBE_THERMAL ("gamma" and
+ * "gamma_bound" are a simple film plus the optional γ / wall fields).
+ * L >= 3. */
+int swalbe1d_create(int32_t L, int32_t dtype, int32_t kind, swalbe_film1d **out);
+int swalbe1d_destroy(swalbe_film1d *st);
+int swalbe1d_length(swalbe_film1d *st, int32_t *L);
+/* bytes = n * 8 (host doubles) or n * 4 (host floats), n = L or 3L */
+int swalbe1d_upload(swalbe_film1d *st, int32_t field, const void *host, size_t bytes);
+int swalbe1d_download(swalbe_film1d *st, int32_t field, void *host, size_t bytes);
+int swalbe1d_fill(swalbe_film1d *st, int32_t field, double value);
+int swalbe1d_synchronize(swalbe_film1d *st);
+
+/* equilibrium!(state::State_1D, sys; g) -- src/equilibrium.jl:169-191 */
+int swalbe1d_equilibrium(swalbe_film1d *st, double g);
+/* BGKandStream!(state, sys) -- src/collide.jl:179-212; bound = 1 is the
+ * bounce-back method of StateWithBound_1D (:214-249) with BORDER1/BORDER2 */
+int swalbe1d_bgk_and_stream(swalbe_film1d *st, double tau, int32_t bound);
+/* moments!(state) -- src/moments.jl:54-76 */
+int swalbe1d_moments(swalbe_film1d *st);
+/* filmpressure!(state, sys; θ, n, m, hmin, hcrit, γ) -- src/pressure.jl:231-315,
+ * raw = 1 the Vector method (:196-229, fast_93 / fast_32, SWALBE_EDOMAIN
+ * otherwise).  use_theta_field / use_gamma_field take θ / γ from the
+ * resident THETA / GAMMA fields (θ and γ broadcast as vectors). */
+int swalbe1d_filmpressure(swalbe_film1d *st, double gamma, double theta, int32_t n, int32_t m,
+                          double hmin, double hcrit, int32_t use_theta_field,
+                          int32_t use_gamma_field, int32_t raw);
+/* ∇γ!(state) (form 0) / ∇γ!(state, sys) (form 1) -- src/forcing.jl:423-447:
+ * GRADGAMMA from GAMMA (and the height, form 1) */
+int swalbe1d_grad_gamma(swalbe_film1d *st, int32_t form, double delta);
+/* h∇p!(state) -- src/forcing.jl:189-198, 223-232 */
+int swalbe1d_hgradp(swalbe_film1d *st);
+/* slippage!(state, sys) -- src/forcing.jl:68-82 */
+int swalbe1d_slippage(swalbe_film1d *st, double delta, double mu);
+/* thermal!(state, sys) -- src/forcing.jl:322-335, randn! replaced by the
+ * counter-based normal of (seed, step, node x) */
+int swalbe1d_thermal(swalbe_film1d *st, double kbt, double mu, double delta, uint64_t seed,
+                     uint64_t step);
+/* inclination!(α, state; t, tstart, tsmooth) -- src/forcing.jl:379-389,
+ * fac = 0.5 + 0.5 tanh((t - tstart)/tsmooth) */
+int swalbe1d_inclination(swalbe_film1d *st, double alpha, double fac);
+/* F .= -h∇p .- slip [.- kbt] [.- ∇γ | .+ ∇γ] (src/simulate.jl:111, 548;
+ * analyze/droplet_coalescence.jl:148 adds ∇γ): grad_gamma -1, 0 or +1 */
+int swalbe1d_force_sum(swalbe_film1d *st, int32_t with_thermal, int32_t grad_gamma);
+/* {Σ, max, min, count(> thresh)} of a field in FP64; use_interior: Σ(h - interior h)
+ * as the walled loop's mass (src/simulate.jl:189) */
+int swalbe1d_reduce(swalbe_film1d *st, int32_t field, double thresh, int32_t use_interior,
+                    double out[4]);
+
+/* ---- fused 1D time loop: time_loop(sys::SysConst_1D, state[, θ | Δh | f,
+ * measure]) and the walled loop (src/simulate.jl:98-205), run_gamma's loop
+ * (:518-560).  Steps run are t0+1 .. t0+nsteps. */
+typedef struct swalbe1d_step_opts {
+    int32_t use_theta_field; /* θ from THETA (src/simulate.jl:118-136) */
+    int32_t use_gamma_field; /* γ from GAMMA (filmpressure!(state, sys, γ = gamma), :545) */
+    int32_t grad_gamma;      /* F ∓ ∇γ from GRADGAMMA: -1 run_gamma (:548), +1 the scans, 0 off */
+    int32_t thermal;         /* F .-= thermal! noise each step */
+    int32_t body;            /* inclination!(α, state) after the force sum */
+    int32_t bound;           /* bounce-back walls (BORDER1/2); mass Σ(h - interior h) */
+    int32_t materialize;     /* afterwards pressure, h∇p, slip, kbt, F, feq hold the last step's */
+    int32_t variant;         /* 0 = resident when the film fits in LDS, else one launch per
+                                step; 1 = one launch per step; 2 = resident (error if too big) */
+    uint64_t seed;
+    int64_t t0;
+    double alpha, body_fac;
+    double wet_thresh;
+    double *dh_series;     /* host [nsteps]: max(h) - min(h) before each step (:147), or NULL */
+    int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) before each step, or NULL */
+    double *mass_series;   /* host: Σh before each step t with t % tdump == 0 (:100-105) */
+    int64_t snap_every;    /* snapshot!(fluid, h, t; dumping) (src/measures.jl:99-105) */
+    double *snap_series;   /* host [nsnap * L], in step order, or NULL */
+} swalbe1d_step_opts;
+int swalbe1d_time_loop(swalbe_film1d *st, const swalbe_params *p, int64_t nsteps,
+                       const swalbe1d_step_opts *opts);
+/* Same, reporting the device time of the loop (HIP events on the film's stream). */
+int swalbe1d_time_loop_timed(swalbe_film1d *st, const swalbe_params *p, int64_t nsteps,
+                             const swalbe1d_step_opts *opts, double *ms_out);
+/* Many 1D films stepped together in one launch, one workgroup per film,
+ * each with its own parameters and options (params[i], opts[i]; opts may be
+ * NULL).  No reference counterpart: the coalescence scans
+ * (analyze/droplet_coalescence.jl) loop one film at a time.  Each film's
+ * result equals its swalbe1d_time_loop with variant 2.  All films share a
+ * dtype and a device and each must fit in LDS. */
+int swalbe1d_ensemble_time_loop(swalbe_film1d *const *films, int32_t nfilms,
+                                const swalbe_params *params, int64_t nsteps,
+                                const swalbe1d_step_opts *opts);
+/* Largest L that runs resident (whole film in LDS) on the current device. */
+int swalbe1d_resident_max_length(int32_t dtype, int32_t tau_is_one, int32_t *L);
+/* ∇f!(output::Vector, f, dgrad[, a]) (src/differences.jl:208-228) and
+ * ∇²f!(output, f::Vector, dgrad) (:77-85) on caller-owned device vectors */
+int swalbe1d_grad_ptr(int32_t dtype, void *out, const void *f, const void *a, int32_t L,
+                      void *stream);
+int swalbe1d_laplacian_ptr(int32_t dtype, void *out, const void *f, int32_t L, void *stream);
+
 /* ---- raw-array twins on caller-owned device arrays (plain values of the
  * given dtype, Julia layout), enqueued on `stream` (NULL = default stream).
  * These replace the positional-array methods the reference exposes. */

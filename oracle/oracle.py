@@ -383,7 +383,8 @@ class LoopOpts1D(C.Structure):
         ("theta_field", _D), ("thermal", C.c_int32), ("body", C.c_int32),
         ("alpha", C.c_double), ("body_fac", C.c_double), ("border1", _D), ("border2", _D),
         ("seed", C.c_uint64), ("t0", C.c_int64), ("dh_series", _D), ("wet_series", _I64),
-        ("wet_thresh", C.c_double), ("mass_series", _D),
+        ("wet_thresh", C.c_double), ("mass_series", _D), ("gamma_field", _D),
+        ("grad_gamma", _D), ("grad_gamma_sign", C.c_int32), ("interior", _D),
     ]
 
 
@@ -436,14 +437,25 @@ def moments_1d(f):
     return h, u
 
 
-def filmpressure_1d(h, gamma, theta, n, m, hmin, hcrit, theta_field=None):
+def filmpressure_1d(h, gamma, theta, n, m, hmin, hcrit, theta_field=None, gamma_field=None):
     h = _f(h)
     L = h.shape[0]
     p = np.zeros(L)
     tf = _f(theta_field, (L,)) if theta_field is not None else None
-    lib().orc1d_filmpressure(_p(p), _p(h), C.c_double(gamma), C.c_double(theta), _p(tf), n, m,
-                             C.c_double(hmin), C.c_double(hcrit), L)
+    gf = _f(gamma_field, (L,)) if gamma_field is not None else None
+    lib().orc1d_filmpressure(_p(p), _p(h), C.c_double(gamma), _p(gf), C.c_double(theta), _p(tf),
+                             n, m, C.c_double(hmin), C.c_double(hcrit), L)
     return p
+
+
+def grad_gamma_1d(gamma, h=None, delta=1.0, form=0):
+    """∇γ!(state) (form 0) or ∇γ!(state, sys) (form 1)."""
+    g = _f(gamma)
+    L = g.shape[0]
+    out = np.zeros(L)
+    hh = _f(h, (L,)) if h is not None else np.ones(L)
+    lib().orc1d_grad_gamma(_p(out), _p(g), _p(hh), C.c_double(delta), int(form), L)
+    return out
 
 
 def filmpressure_raw_1d(h, gamma, theta, n, m, hmin, hcrit):
@@ -516,7 +528,8 @@ def obslist_1d(obs):
 
 
 def time_loop_1d(state: State1D, prm: Params, nsteps: int, *, theta_field=None, thermal=False,
-                 seed=0, t0=0, body=None, borders=None, dh=False, wetted=None, mass=False):
+                 seed=0, t0=0, body=None, borders=None, dh=False, wetted=None, mass=False,
+                 gamma_field=None, grad_gamma=None, grad_gamma_sign=-1, interior=None):
     """The 1D time loop; body = (α, fac); borders = (border1, border2)."""
     o = LoopOpts1D()
     keep = []
@@ -524,6 +537,18 @@ def time_loop_1d(state: State1D, prm: Params, nsteps: int, *, theta_field=None, 
         tf = _f(theta_field, (state.L,))
         keep.append(tf)
         o.theta_field = _p(tf)
+    if gamma_field is not None:
+        gf = _f(gamma_field, (state.L,))
+        keep.append(gf)
+        o.gamma_field = _p(gf)
+    if grad_gamma is not None:
+        gg = _f(grad_gamma, (state.L,))
+        keep.append(gg)
+        o.grad_gamma, o.grad_gamma_sign = _p(gg), int(grad_gamma_sign)
+    if interior is not None:
+        it = _f(interior, (state.L,))
+        keep.append(it)
+        o.interior = _p(it)
     o.thermal, o.seed, o.t0 = int(thermal), seed, t0
     if body is not None:
         o.body, o.alpha, o.body_fac = 1, body[0], body[1]

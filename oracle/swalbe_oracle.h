@@ -124,15 +124,21 @@ typedef struct orc1d_loop_opts {
     int64_t *wet_series;
     double wet_thresh;
     double *mass_series;
+    const double *gamma_field; /* γ field in filmpressure! (run_gamma) */
+    const double *grad_gamma;  /* ∇γ field, applied to F with grad_gamma_sign */
+    int32_t grad_gamma_sign;   /* -1: F .-= ∇γ, +1: F .+= ∇γ */
+    const double *interior;    /* wall interior mask for the mass, or NULL */
 } orc1d_loop_opts;
 
 void orc1d_equilibrium(double *feq, const double *h, const double *u, double g, int L);
 void orc1d_bgk_and_stream(double *fout, const double *feq, double *ftemp, const double *F,
                           double tau, const double *border1, const double *border2, int L);
 void orc1d_moments(double *h, double *u, const double *f, int L);
-void orc1d_filmpressure(double *p, const double *h, double gamma, double theta,
-                        const double *theta_field, int n, int m, double hmin, double hcrit,
-                        int L);
+void orc1d_filmpressure(double *p, const double *h, double gamma, const double *gamma_field,
+                        double theta, const double *theta_field, int n, int m, double hmin,
+                        double hcrit, int L);
+void orc1d_grad_gamma(double *out, const double *gam, const double *h, double delta, int form,
+                      int L);
 int orc1d_filmpressure_raw(double *p, const double *h, double gamma, double theta, int n, int m,
                            double hmin, double hcrit, int L);
 void orc1d_grad(double *out, const double *f, const double *a, int L);

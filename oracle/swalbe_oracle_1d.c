@@ -84,19 +84,39 @@ static inline double kappa1d(double theta, int n, int m, double hmin) {
 
 /* filmpressure!(state::LBM_state_1D, sys; θ, n, m, hmin, hcrit, γ):
  * src/pressure.jl:231-255 (power_broad form); θ may be a field
- * (run_dropletpatterned 1D, src/simulate.jl:442-460).
- * Pinned: test/pressure.jl:56-103. */
-void orc1d_filmpressure(double *p, const double *h, double gamma, double theta,
-                        const double *theta_field, int n, int m, double hmin, double hcrit,
-                        int L) {
+ * (run_dropletpatterned 1D, src/simulate.jl:442-460) and so may γ (the
+ * State_gamma_1D method, src/pressure.jl:284-315, called with γ = gamma by
+ * run_gamma, src/simulate.jl:545).  gamma_field == NULL uses the scalar.
+ * Pinned: test/pressure.jl:56-103; a constant γ field reproduces the scalar. */
+void orc1d_filmpressure(double *p, const double *h, double gamma, const double *gamma_field,
+                        double theta, const double *theta_field, int n, int m, double hmin,
+                        double hcrit, int L) {
     for (int x = 0; x < L; ++x) {
         const double th = theta_field ? theta_field[x] : theta;
+        const double gm = gamma_field ? gamma_field[x] : gamma;
         const double a = hmin / (h[x] + hcrit);
-        p[x] = -gamma * (kappa1d(th, n, m, hmin) * (orc_power_broad(a, n) - orc_power_broad(a, m)));
+        p[x] = -gm * (kappa1d(th, n, m, hmin) * (orc_power_broad(a, n) - orc_power_broad(a, m)));
     }
     for (int x = 0; x < L; ++x) {
+        const double gm = gamma_field ? gamma_field[x] : gamma;
         const double hip = h[(x - 1 + L) % L], him = h[(x + 1) % L];
-        p[x] = p[x] - gamma * ((hip - 2.0 * h[x]) + him);
+        p[x] = p[x] - gm * ((hip - 2.0 * h[x]) + him);
+    }
+}
+
+/* ∇γ!(state) (src/forcing.jl:423-432): -3/2 (γ[x-1] - γ[x+1]) / 2 (form 0), and
+ * ∇γ!(state, sys) (:434-447): (2h² + 6δh + 3δ²)/(6h) · h/2 · (γ[x-1] - γ[x+1])/2
+ * (form 1).  Pinned by a linear γ ramp (exact slope) and a constant γ (zero). */
+void orc1d_grad_gamma(double *out, const double *gam, const double *h, double delta, int form,
+                      int L) {
+    for (int x = 0; x < L; ++x) {
+        const double fip = gam[(x - 1 + L) % L], fim = gam[(x + 1) % L];
+        if (form == 0) {
+            out[x] = -1.5 * ((fip - fim) / 2.0);
+        } else {
+            const double a = ((2.0 * (h[x] * h[x])) + ((6.0 * delta) * h[x])) + 3.0 * (delta * delta);
+            out[x] = (((a / (6.0 * h[x])) * h[x]) / 2.0) * ((fip - fim) / 2.0);
+        }
     }
 }
 
@@ -200,8 +220,10 @@ int orc1d_time_loop(orc1d_state *st, const orc_params *p, int64_t nsteps,
     for (int64_t it = 1; it <= nsteps; ++it) {
         const int64_t t = o->t0 + it;
         if (o->mass_series && p->tdump > 0 && t % p->tdump == 0) {
+            /* Σ(h - interior h) with walls (src/simulate.jl:189), else Σh (:101) */
             double s = 0.0;
-            for (int x = 0; x < L; ++x) s += st->height[x];
+            for (int x = 0; x < L; ++x)
+                s += o->interior ? st->height[x] - o->interior[x] * st->height[x] : st->height[x];
             o->mass_series[km++] = s;
         }
         if (o->dh_series) {
@@ -212,13 +234,19 @@ int orc1d_time_loop(orc1d_state *st, const orc_params *p, int64_t nsteps,
             }
             o->dh_series[it - 1] = mx - mn;
         }
-        orc1d_filmpressure(st->pressure, st->height, p->gamma, p->theta, o->theta_field, p->n,
-                           p->m, p->hmin, p->hcrit, L);
+        orc1d_filmpressure(st->pressure, st->height, p->gamma, o->gamma_field, p->theta,
+                           o->theta_field, p->n, p->m, p->hmin, p->hcrit, L);
         orc1d_hgradp(st->hgp, st->pressure, st->height, L);
         orc1d_slippage(st->slip, st->height, st->vel, p->delta, p->mu, L);
         if (o->thermal)
             orc1d_thermal(st->kbt, st->height, p->kbt, p->mu, p->delta, o->seed, (uint64_t)t, L);
         orc1d_force_sum(st->F, st->hgp, st->slip, o->thermal ? st->kbt : NULL, L);
+        /* run_gamma: F .= -h∇p .- slip .- ∇γ (src/simulate.jl:548); the
+         * coalescence scans add it (analyze/droplet_coalescence.jl:148) */
+        if (o->grad_gamma && o->grad_gamma_sign < 0)
+            for (int x = 0; x < L; ++x) st->F[x] = st->F[x] - o->grad_gamma[x];
+        if (o->grad_gamma && o->grad_gamma_sign > 0)
+            for (int x = 0; x < L; ++x) st->F[x] = st->F[x] + o->grad_gamma[x];
         if (o->wet_series) {
             int64_t c = 0;
             for (int x = 0; x < L; ++x) c += st->height[x] > o->wet_thresh ? 1 : 0;

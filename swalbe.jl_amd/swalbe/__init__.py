@@ -1,4 +1,4 @@
-"""swalbe -- Swalbe.jl's 2D thin-film lattice Boltzmann step on MI355X.
+"""swalbe -- Swalbe.jl's thin-film lattice Boltzmann step (2D D2Q9, 1D D1Q3) on MI355X.
 
 The Python face of libswalbe_hip.so: the reference's parameter structs, a
 GPU-resident state and the ops / time loop / run_* drivers by the
@@ -21,7 +21,10 @@ from .batch import (INIT_CAP, INIT_FLAT, INIT_NORMAL, INIT_TWO_CAPS, Batch, Film
                     film_bytes, film_init)
 from .io import load_state, restart_from_height, save_snapshots, save_state, snapshot
 from .simulate import (run_dropletforced, run_dropletpatterned, run_dropletrelax, run_flat,
-                       run_random, run_rayleightaylor)
+                       run_gamma, run_random, run_rayleightaylor)
+from . import one_d
+from .one_d import (HipState1D, SysConst_1D, SysConstWithBound_1D, ensemble_time_loop, grad_gamma,
+                    obslist, obslist1D, resident_max_length, two_droplets)
 
 lib()  # the HIP library is required: fail at import if it is missing
 

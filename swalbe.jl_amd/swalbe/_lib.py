@@ -22,6 +22,12 @@ FIELDS = {
 }
 DIST_FIELDS = ("fout", "ftemp", "feq")
 
+FIELDS_1D = {
+    "fout": 0, "ftemp": 1, "feq": 2, "height": 3, "vel": 4, "pressure": 5, "F": 6, "slip": 7,
+    "hgradp": 8, "kbt": 9, "theta": 10, "gamma": 11, "gradgamma": 12, "border1": 13,
+    "border2": 14, "interior": 15,
+}
+
 
 class SwalbeError(RuntimeError):
     """A HIP runtime, allocation or argument error reported by the library;
@@ -55,6 +61,19 @@ class StepOpts(C.Structure):
         ("wet_series", C.POINTER(C.c_int64)), ("mass_series", C.POINTER(C.c_double)),
         ("tile_y", C.c_int32), ("variant", C.c_int32), ("nontemporal", C.c_int32),
         ("pad_", C.c_int32), ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
+    ]
+
+
+class StepOpts1D(C.Structure):
+    """swalbe1d_step_opts: options of the fused 1D time loop."""
+    _fields_ = [
+        ("use_theta_field", C.c_int32), ("use_gamma_field", C.c_int32), ("grad_gamma", C.c_int32),
+        ("thermal", C.c_int32), ("body", C.c_int32), ("bound", C.c_int32),
+        ("materialize", C.c_int32), ("variant", C.c_int32), ("seed", C.c_uint64),
+        ("t0", C.c_int64), ("alpha", C.c_double), ("body_fac", C.c_double),
+        ("wet_thresh", C.c_double), ("dh_series", C.POINTER(C.c_double)),
+        ("wet_series", C.POINTER(C.c_int64)), ("mass_series", C.POINTER(C.c_double)),
+        ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
     ]
 
 
@@ -164,6 +183,32 @@ def lib():
     _sig(L, "swalbe_memcpy_d2h", [_V, _V, C.c_size_t])
     _sig(L, "swalbe_host_normals", [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, C.POINTER(_D), C.POINTER(_D)])
     _sig(L, "swalbe_host_uniforms", [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, C.POINTER(_D)])
+    _sig(L, "swalbe1d_create", [_I, _I, _I, C.POINTER(_V)])
+    _sig(L, "swalbe1d_destroy", [_V])
+    _sig(L, "swalbe1d_length", [_V, C.POINTER(_I)])
+    _sig(L, "swalbe1d_upload", [_V, _I, _V, C.c_size_t])
+    _sig(L, "swalbe1d_download", [_V, _I, _V, C.c_size_t])
+    _sig(L, "swalbe1d_fill", [_V, _I, _D])
+    _sig(L, "swalbe1d_synchronize", [_V])
+    _sig(L, "swalbe1d_equilibrium", [_V, _D])
+    _sig(L, "swalbe1d_bgk_and_stream", [_V, _D, _I])
+    _sig(L, "swalbe1d_moments", [_V])
+    _sig(L, "swalbe1d_filmpressure", [_V, _D, _D, _I, _I, _D, _D, _I, _I, _I])
+    _sig(L, "swalbe1d_grad_gamma", [_V, _I, _D])
+    _sig(L, "swalbe1d_hgradp", [_V])
+    _sig(L, "swalbe1d_slippage", [_V, _D, _D])
+    _sig(L, "swalbe1d_thermal", [_V, _D, _D, _D, C.c_uint64, C.c_uint64])
+    _sig(L, "swalbe1d_inclination", [_V, _D, _D])
+    _sig(L, "swalbe1d_force_sum", [_V, _I, _I])
+    _sig(L, "swalbe1d_reduce", [_V, _I, _D, _I, C.POINTER(_D)])
+    _sig(L, "swalbe1d_time_loop", [_V, C.POINTER(Params), C.c_int64, C.POINTER(StepOpts1D)])
+    _sig(L, "swalbe1d_time_loop_timed",
+         [_V, C.POINTER(Params), C.c_int64, C.POINTER(StepOpts1D), C.POINTER(_D)])
+    _sig(L, "swalbe1d_ensemble_time_loop",
+         [C.POINTER(_V), _I, C.POINTER(Params), C.c_int64, C.POINTER(StepOpts1D)])
+    _sig(L, "swalbe1d_resident_max_length", [_I, _I, C.POINTER(_I)])
+    _sig(L, "swalbe1d_grad_ptr", [_I, _V, _V, _V, _I, _V])
+    _sig(L, "swalbe1d_laplacian_ptr", [_I, _V, _V, _I, _V])
     L.swalbe_host_cospi.restype = C.c_double
     L.swalbe_host_cospi.argtypes = [C.c_double]
     _lib = L

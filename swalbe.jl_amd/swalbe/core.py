@@ -17,6 +17,15 @@ import numpy as np
 from . import _lib
 from ._lib import DIST_FIELDS, FIELDS, check, lib
 
+def _one_d():
+    from . import one_d
+    return one_d
+
+
+def _is1d(x) -> bool:
+    return getattr(x, "_is_1d", False)
+
+
 # ---------------------------------------------------------------- parameters
 
 
@@ -178,7 +187,10 @@ class HipState:
 
 def Sys(sysc: SysConst, device: str = "GPU", T=np.float64, kind: str = "simple") -> HipState:
     """Allocates a state like Sys(sysc, device; T, kind) (src/initialize.jl:491-572):
-    all fields zero and height = 1, resident on the GPU."""
+    all fields zero and height = 1, resident on the GPU.  A SysConst_1D gives
+    the 1D state of Sys(sysc::Consts_1D; T, kind) (:587-616)."""
+    if isinstance(sysc, _one_d().SysConst_1D):
+        return _one_d().Sys(sysc, T=T, kind=kind)
     if device not in ("GPU", "HIP"):
         raise ValueError("this package implements the GPU device (device = 'GPU' or 'HIP')")
     return HipState(sysc.Lx, sysc.Ly, T=T, kind=kind)
@@ -269,6 +281,8 @@ def _shape2(a):
 def equilibrium(*args, **kw):
     """equilibrium!(state, sys) or equilibrium!(feq, h, ux, uy, vsq, g)
     (src/equilibrium.jl:63-128)."""
+    if _is1d(args[0]):
+        return _one_d().equilibrium(*args, **kw)
     if isinstance(args[0], HipState):
         state, sys = args[0], args[1]
         check(lib().swalbe_equilibrium(state.handle, float(kw.get("g", sys.param.g))), "equilibrium")
@@ -285,6 +299,8 @@ def equilibrium(*args, **kw):
 def BGKandStream(*args, τ=None, tau=None):
     """BGKandStream!(state, sys) or BGKandStream!(fout, feq, ftemp, Fx, Fy, τ)
     (src/collide.jl:70-161)."""
+    if _is1d(args[0]):
+        return _one_d().BGKandStream(args[0], args[1], τ=τ if τ is not None else tau)
     if isinstance(args[0], HipState):
         state, sys = args[0], args[1]
         t = τ if τ is not None else (tau if tau is not None else sys.param.τ)
@@ -301,6 +317,8 @@ def BGKandStream(*args, τ=None, tau=None):
 
 def moments(*args):
     """moments!(state) or moments!(height, velx, vely, fout) (src/moments.jl:43-73)."""
+    if _is1d(args[0]):
+        return _one_d().moments(args[0])
     if isinstance(args[0], HipState):
         check(lib().swalbe_moments(args[0].handle), "moments")
         return None
@@ -321,6 +339,8 @@ def filmpressure(*args, θ=None, γ=None, n=None, m=None, hmin=None, hcrit=None,
     γ = ascii_kw.pop("gamma", γ)
     if ascii_kw:
         raise TypeError(f"unknown keywords {sorted(ascii_kw)}")
+    if _is1d(args[0]):
+        return _one_d().filmpressure(args[0], args[1], θ=θ, γ=γ, n=n, m=m, hmin=hmin, hcrit=hcrit)
     if isinstance(args[0], HipState):
         state, sys = args[0], args[1]
         P = sys.param
@@ -352,7 +372,9 @@ def filmpressure(*args, θ=None, γ=None, n=None, m=None, hmin=None, hcrit=None,
 
 def grad_f(outx, outy, f, *rest):
     """∇f!(outx, outy, f), ∇f!(outx, outy, f, a), ∇f!(outx, outy, f, dgrad, a)
-    (src/differences.jl:153-206)."""
+    (src/differences.jl:153-206); on vectors the 1D ∇f!(output, f, dgrad[, a])."""
+    if np.ndim(outx) == 1:
+        return _one_d().grad_f(outx, outy, f, rest[0] if rest else None)
     a = rest[-1] if rest else None
     s2 = _shape2(f)
     R = _Raw(_dt([outx, outy, f]), s2)
@@ -362,7 +384,10 @@ def grad_f(outx, outy, f, *rest):
 
 
 def laplacian_f(output, f, γ):
-    """∇²f!(output, f, γ) (src/differences.jl:57-75)."""
+    """∇²f!(output, f, γ) (src/differences.jl:57-75); on vectors the 1D
+    ∇²f!(output, f, dgrad) (:77-85), whose third argument is scratch."""
+    if np.ndim(output) == 1:
+        return _one_d().laplacian_f(output, f)
     s2 = _shape2(f)
     R = _Raw(_dt([output, f]), s2)
     check(lib().swalbe_laplacian_ptr(R.dtype, R.arg(output, s2, True), R.arg(f, s2), float(γ), s2[0],
@@ -372,6 +397,8 @@ def laplacian_f(output, f, γ):
 
 def h_grad_p(state: HipState):
     """h∇p!(state) (src/forcing.jl:168-221)."""
+    if _is1d(state):
+        return _one_d().h_grad_p(state)
     check(lib().swalbe_hgradp(state.handle), "h_grad_p")
 
 
@@ -386,6 +413,8 @@ def h_grad_p_raw(hgpx, hgpy, p, h):
 
 def slippage(*args):
     """slippage!(state, sys) or slippage!(slipx, slipy, h, ux, uy, δ, μ) (src/forcing.jl:42-66)."""
+    if _is1d(args[0]):
+        return _one_d().slippage(*args)
     if isinstance(args[0], HipState):
         state, sys = args
         check(lib().swalbe_slippage(state.handle, sys.param.δ, sys.param.μ), "slippage")
@@ -403,6 +432,8 @@ def thermal(*args, seed: int = 0, step: Optional[int] = None):
     """thermal!(state, sys) or thermal!(fx, fy, h, kbt, μ, δ) (src/forcing.jl:297-320).
     randn! is replaced by normals keyed (seed, step, node); step defaults to
     the state's step counter."""
+    if _is1d(args[0]):
+        return _one_d().thermal(args[0], args[1], seed=seed, step=step)
     if isinstance(args[0], HipState):
         state, sys = args
         st = state.t if step is None else step
@@ -423,13 +454,18 @@ def _incl_fac(t, tstart, tsmooth):
 
 
 def inclination(α, state, t: int = 1000, tstart: int = 0, tsmooth: int = 1):
-    """inclination!(α, state; t, tstart, tsmooth) (src/forcing.jl:363-377)."""
+    """inclination!(α, state; t, tstart, tsmooth) (src/forcing.jl:363-377);
+    α a number for a 1D state (:379-389)."""
+    if _is1d(state):
+        return _one_d().inclination(α, state, t=t, tstart=tstart, tsmooth=tsmooth)
     check(lib().swalbe_inclination(state.handle, float(α[0]), float(α[1]),
                                    _incl_fac(t, tstart, tsmooth)), "inclination")
 
 
 def force_sum(state: HipState, thermal: bool = False):
     """state.Fx .= -state.h∇px .- state.slipx [.- state.kbtx] (src/simulate.jl:18-19)."""
+    if _is1d(state):
+        return _one_d().force_sum(state, thermal)
     check(lib().swalbe_force_sum(state.handle, int(bool(thermal))), "force_sum")
 
 
@@ -465,7 +501,7 @@ def wetted(area_size, state, hthresh: float = 0.055):
 
 def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, thermal: bool = False,
               seed: int = 0, materialize: bool = True, tile_y: int = 0, mass: Optional[list] = None,
-              snapshots: Optional[dict] = None, snapshot_every: int = 0):
+              snapshots: Optional[dict] = None, snapshot_every: int = 0, **kw1d):
     """time_loop(sys, state), time_loop(sys, state, θ), time_loop(sys, state, Δh)
     and time_loop(sys, state, f, measure) (src/simulate.jl:6-96).
 
@@ -477,7 +513,13 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     (scripts/Rivulet_stability.jl:123-124).  `snapshots` (a dict) receives
     "h_<t>" for every step t divisible by `snapshot_every`, as snapshot!
     (src/measures.jl:99-105) would record them, copied off the device while
-    the following steps run."""
+    the following steps run.  A 1D state runs the 1D loop (its own keywords:
+    variant, γ, grad_gamma_sign, snapshots + dumping)."""
+    if _is1d(state):
+        return _one_d().time_loop(sys, state, *args, verbose=verbose, thermal=thermal, seed=seed,
+                                  materialize=materialize, mass=mass, snapshots=snapshots, **kw1d)
+    if kw1d:
+        raise TypeError(f"unknown keywords {sorted(kw1d)}")
     P = sys.param
     prm = P.cparams()
     nsteps = P.Tmax

@@ -52,6 +52,9 @@ def uniforms(n: int, seed: int, step: int = 0, tag: int = TAG_INIT_UNIFORM):
 def randinterface(height: np.ndarray, h0: float, eps: float, seed: int = 0) -> None:
     """randinterface!(height, h₀, ϵ) (src/initialvalues.jl:23-33):
     height .= h₀ (1 + ϵ N(0,1)), normals from the counter stream."""
+    if height.ndim == 1:
+        from .one_d import randinterface as r1
+        return r1(height, h0, eps, seed=seed)
     z0, _ = lattice_normals(height.shape[0], height.shape[1], seed)
     height[...] = h0 * (1.0 + eps * z0)
 
@@ -69,7 +72,11 @@ def singledroplet(height: np.ndarray, radius, θ, center, precursor: float = 0.0
     """singledroplet(height, radius, θ, center) (src/initialvalues.jl:203-224):
     a spherical cap h = (cos(asin(r/R)) - cos πθ) R, precursor elsewhere.
     `center` is 1-based as in the reference; the precursor (0.05 there) is a
-    parameter here."""
+    parameter here.  A vector height is the 1D method (hcrit clamp, scalar
+    center)."""
+    if height.ndim == 1:
+        from .one_d import singledroplet as s1
+        return s1(height, radius, θ, center, hcrit=precursor)
     lx, ly = height.shape
     i = np.arange(1, lx + 1)[:, None]
     j = np.arange(1, ly + 1)[None, :]

@@ -1,15 +1,18 @@
-"""The 2D run_* drivers of src/simulate.jl:236-484 on a GPU-resident state."""
+"""The run_* drivers of src/simulate.jl:236-560 on a GPU-resident state; a
+SysConst_1D selects the 1D driver of the same name."""
 from __future__ import annotations
 
 import numpy as np
 
-from . import core
+from . import core, one_d
 from .core import Sys, SysConst, equilibrium, inclination, time_loop, wetted
 from .initialvalues import randinterface, sinewave, singledroplet
 
 
 def run_flat(sys: SysConst, device: str = "GPU", verbos: bool = True, T=np.float64):
     """run_flat (src/simulate.jl:236-245): a flat film stays flat."""
+    if isinstance(sys, one_d.SysConst_1D):
+        return one_d.run_flat(sys, verbos=verbos, T=T)
     print("Simulating a flat interface without driving forces (nothing should happen) in two dimensions")
     state = Sys(sys, device, T=T)
     state.height = 1.0
@@ -20,6 +23,8 @@ def run_flat(sys: SysConst, device: str = "GPU", verbos: bool = True, T=np.float
 def run_random(sys: SysConst, device: str = "GPU", h0: float = 1.0, eps: float = 0.01,
                verbos: bool = True, seed: int = 0, T=np.float64):
     """run_random (src/simulate.jl:286-294): a randomly undulated film."""
+    if isinstance(sys, one_d.SysConst_1D):
+        return one_d.run_random(sys, h0=h0, eps=eps, verbos=verbos, seed=seed, T=T)
     print("Simulating a random undulated interface in two dimensions")
     state = Sys(sys, device, T=T)
     h = np.ones((sys.Lx, sys.Ly), order="F")
@@ -31,8 +36,10 @@ def run_random(sys: SysConst, device: str = "GPU", h0: float = 1.0, eps: float =
 
 
 def run_rayleightaylor(sys: SysConst, device: str = "GPU", kx: int = 15, ky: int = 18, h0: float = 1.0,
-                       eps: float = 0.001, verbos: bool = True, T=np.float64):
+                       eps: float = 0.001, verbos: bool = True, T=np.float64, **kw):
     """run_rayleightaylor (src/simulate.jl:338-358): returns (height, Δh series)."""
+    if isinstance(sys, one_d.SysConst_1D):
+        return one_d.run_rayleightaylor(sys, h0=h0, eps=eps, verbos=verbos, T=T, **kw)
     print("Simulating the Rayleigh Taylor instability in two dimensions")
     state = Sys(sys, device, T=T)
     state.height = sinewave(sys.Lx, sys.Ly, h0, kx, ky, eps)
@@ -45,6 +52,8 @@ def run_rayleightaylor(sys: SysConst, device: str = "GPU", kx: int = 15, ky: int
 def run_dropletrelax(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, center=None,
                      verbos: bool = True, T=np.float64):
     """run_dropletrelax (src/simulate.jl:384-399): returns (height, wetted area series)."""
+    if isinstance(sys, one_d.SysConst_1D):
+        return one_d.run_dropletrelax(sys, radius=radius, θ0=θ0, center=center, verbos=verbos, T=T)
     print("Simulating an out of equilibrium droplet in two dimensions")
     center = center or (sys.Lx // 2, sys.Ly // 2)
     state = Sys(sys, device, T=T)
@@ -58,6 +67,9 @@ def run_dropletrelax(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, c
 def run_dropletpatterned(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, center=None,
                          θs=None, verbos: bool = True, T=np.float64):
     """run_dropletpatterned (src/simulate.jl:422-439): contact-angle field θₛ."""
+    if isinstance(sys, one_d.SysConst_1D):
+        return one_d.run_dropletpatterned(sys, radius=radius, θ0=θ0, center=center, θs=θs,
+                                          verbos=verbos, T=T)
     print("Simulating a droplet on a patterned substrate in two dimensions")
     center = center or (sys.Lx // 2, sys.Ly // 2)
     θs = np.full((sys.Lx, sys.Ly), 1 / 9) if θs is None else θs
@@ -69,9 +81,12 @@ def run_dropletpatterned(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 
 
 
 def run_dropletforced(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, center=None, fx=0.0,
-                      fy=0.0, verbos: bool = True, T=np.float64):
+                      fy=0.0, verbos: bool = True, T=np.float64, **kw):
     """run_dropletforced (src/simulate.jl:462-484): a droplet under a body force;
     returns (height, velx, vely)."""
+    if isinstance(sys, one_d.SysConst_1D):
+        return one_d.run_dropletforced(sys, radius=radius, θ0=θ0, center=center, verbos=verbos,
+                                       T=T, **kw)
     bodyforce = [fx, fy]
     print("Simulating a sliding droplet in two dimensions")
     center = center or (sys.Lx // 2, sys.Ly // 2)
@@ -83,5 +98,7 @@ def run_dropletforced(sys: SysConst, device: str = "GPU", radius=20, θ0=1 / 6, 
     return state.height, state.velx, state.vely
 
 
+run_gamma = one_d.run_gamma
+
 __all__ = ["run_flat", "run_random", "run_rayleightaylor", "run_dropletrelax",
-           "run_dropletpatterned", "run_dropletforced", "core"]
+           "run_dropletpatterned", "run_dropletforced", "run_gamma", "core"]

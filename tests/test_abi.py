@@ -17,7 +17,7 @@ def declared_functions():
         if fn.endswith(".h"):
             src = open(os.path.join(ROOT, "include", fn)).read()
             src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-            names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(swalbe_\w+)\s*\(", src, flags=re.M)
+            names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(swalbe(?:1d)?_\w+)\s*\(", src, flags=re.M)
     return sorted(set(names))
 
 

@@ -118,3 +118,61 @@ def test_rayleigh_taylor_1d_grows():
     st.feq[...] = O.equilibrium_1d(st.height, st.vel, -0.002)
     out = O.time_loop_1d(st, O.params(g=-0.002, tdump=500), 1000, dh=True)
     assert out["dh"][0] < out["dh"][-1]
+
+
+def test_gamma_field_pressure_reduces_to_the_scalar_form():
+    # a constant γ field is the scalar γ of the KATs; a γ field scales both
+    # terms node by node
+    rng = np.random.default_rng(3)
+    h = rng.uniform(0.2, 1.5, 50)
+    p_s = O.filmpressure_1d(h, 0.01, 1 / 9, 9, 3, 0.1, 0.05)
+    p_f = O.filmpressure_1d(h, 0.0, 1 / 9, 9, 3, 0.1, 0.05, gamma_field=np.full(50, 0.01))
+    assert np.array_equal(p_s, p_f)
+    gam = rng.uniform(0.005, 0.02, 50)
+    p_g = O.filmpressure_1d(h, 0.0, 1 / 9, 9, 3, 0.1, 0.05, gamma_field=gam)
+    p_1 = O.filmpressure_1d(h, 1.0, 1 / 9, 9, 3, 0.1, 0.05)
+    assert np.allclose(p_g, gam * p_1, rtol=1e-13, atol=1e-18)
+
+
+def test_grad_gamma_forms_on_a_ramp():
+    # ∇γ!(state): -3/2 (γ[x-1] - γ[x+1]) / 2 = 3/2 · slope inside a linear ramp;
+    # ∇γ!(state, sys): (2h² + 6δh + 3δ²)/(6h) · h/2 · slope
+    L = 40
+    x = np.arange(L, dtype=float)
+    gam = 0.01 + 1e-4 * x
+    g0 = O.grad_gamma_1d(gam, form=0)
+    assert np.allclose(g0[1:-1], 1.5 * 1e-4, rtol=1e-9)
+    # the periodic wrap sees the jump at the ends
+    assert g0[0] == -1.5 * ((gam[-1] - gam[1]) / 2.0)
+    assert np.all(O.grad_gamma_1d(np.full(L, 0.02), form=0) == 0.0)
+    h = np.full(L, 0.7)
+    d = 1.0
+    g1 = O.grad_gamma_1d(gam, h, delta=d, form=1)
+    want = (2 * 0.49 + 6 * d * 0.7 + 3 * d * d) / (6 * 0.7) * 0.7 / 2 * (-1e-4)
+    assert np.allclose(g1[1:-1], want, rtol=1e-9)
+
+
+def test_grad_gamma_force_sign_drives_the_flow():
+    # uniform γ in the pressure, a γ step only in ∇γ: from rest on a flat
+    # film the first step's velocity is the ∇γ force, so the two signs of
+    # the force term (run_gamma subtracts it, the coalescence scans add it)
+    # give opposite velocities where γ jumps and none elsewhere
+    L = 64
+    prm = O.params(gamma=0.01, theta=1 / 9, n=9, m=3, hmin=0.1, hcrit=0.05, tdump=1000)
+    gam = np.where(np.arange(L) < L // 2, 0.01, 0.02)
+    gg = O.grad_gamma_1d(gam, form=0)
+    assert gg[L // 2 - 1] > 0 and gg[L // 2] > 0
+    vel = {}
+    for sign in (-1, 1):
+        st = O.State1D(L)
+        st.feq[...] = O.equilibrium_1d(st.height, st.vel, 0.0)
+        st.ftemp[...] = st.feq
+        st.fout[...] = st.feq
+        O.time_loop_1d(st, prm, 1, grad_gamma=gg, grad_gamma_sign=sign)
+        vel[sign] = st.vel.copy()
+        assert abs(st.height.sum() - L) < 1e-12
+    j = L // 2
+    assert vel[-1][j] < 0 < vel[1][j]
+    assert np.allclose(vel[-1], -vel[1], rtol=2e-2, atol=1e-15)
+    far = np.r_[4:j - 4, j + 5:L - 4]
+    assert np.all(vel[-1][far] == 0.0)
