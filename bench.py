@@ -32,8 +32,9 @@ WORKLOADS = {
     "flat4096_f32_nonoise": (4096, 4096, "f32", False),
     "droplet1024_f32": (1024, 1024, "f32", False),
 }
-# many-film workloads (FP32 batched kernel) and the copy calibration
-BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson", "copy4096_f32", "pattern4096_f32")
+# many-film workloads (FP32 batched kernel), the 1D coalescence scan and the copy calibration
+BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson", "copy4096_f32", "pattern4096_f32",
+                   "coalesce1d_scan", "single1d_f64")
 
 
 def bytes_per_lup(esz: int, tau_is_one: bool) -> int:
@@ -107,6 +108,138 @@ def bench_batch(a):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_lup": bpl},
     }
+    print(json.dumps(line), flush=True)
+
+
+# FP64 operations per node and step of the 1D resident kernel in the scan
+# configuration (γ field, ∇γ force, n = 9, m = 3, τ = 1), counted from the
+# source: pressure 19 + 1 division, force 12 + 1 division, collide and
+# stream 33, moments 3 + 1 division (a division counted as one operation)
+FLOP_PER_LUP_1D = 70
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _scan_film(S, k, L, steps):
+    """Film k of the 1D coalescence scan (analyze/droplet_coalescence.jl):
+    two droplets r = 500, θ = 1/9 on L = 1024, γ₀ = 1e-4 with a tanh step of
+    20 % and width sl, slip δ, hmin; every other film has walls at the ends."""
+    import numpy as np
+    sls = [1, 2, 5, 10, 20, 50, 100, 200]
+    sl = sls[k % len(sls)]
+    delta = (11.0, 15.0)[(k // len(sls)) % 2]
+    hmin = (0.1, 0.14)[(k // (2 * len(sls))) % 2]
+    g0 = 1e-4
+    l = np.arange(1.0, L + 1)
+    smooth = np.abs(1 - (0.5 + 0.5 * np.tanh((l - L // 2) / sl)))
+    gamma = g0 * smooth + (1 - smooth) * (g0 - g0 * 20 / 100)
+    param = S.Taumucs(Tmax=steps, tdump=10 ** 9, γ=g0, δ=delta, n=9, m=3, hmin=hmin, θ=1 / 9)
+    if k % 2:
+        obs = np.zeros(L)
+        obs[:5] = 1
+        obs[-5:] = 1
+        sysc = S.SysConstWithBound_1D(L=L, param=param, obs=obs)
+        S.obslist(sysc)
+    else:
+        sysc = S.SysConst_1D(L=L, param=param)
+    st = S.Sys(sysc, T=np.float64, kind="gamma")
+    st.height = S.two_droplets(sysc, r1=500, r2=500, θ1=1 / 9, θ2=1 / 9)
+    S.equilibrium(st, sysc)
+    st.set("gamma", gamma)
+    S.grad_gamma(st)
+    gg = st.get("gradgamma")
+    gg[-5:] = 0.0
+    gg[:4] = 0.0  # state.∇γ[L-4:L] .= 0; state.∇γ[1:4] .= 0
+    st.set("gradgamma", gg)
+    return sysc, st, gamma
+
+
+def cpu_baseline_1d(L=1024, seconds_target=10.0):
+    """The oracle's FP64 1D step (reference-order sweeps, 1 thread) on one scan film."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    import swalbe as S
+    sysc = S.SysConst_1D(L=L, param=S.Taumucs())
+    h = S.two_droplets(sysc, r1=500, r2=500, θ1=1 / 9, θ2=1 / 9)
+    st = O.State1D(L)
+    st.height[...] = h
+    st.feq[...] = O.equilibrium_1d(h, np.zeros(L), 0.0)
+    l = np.arange(1.0, L + 1)
+    smooth = np.abs(1 - (0.5 + 0.5 * np.tanh((l - L // 2) / 10)))
+    gamma = 1e-4 * smooth + (1 - smooth) * 0.8e-4
+    gg = O.grad_gamma_1d(gamma, form=0)
+    prm = O.params(gamma=1e-4, delta=11.0, tdump=10 ** 9)
+    kw = dict(gamma_field=gamma, grad_gamma=gg, grad_gamma_sign=1)
+    t = time.perf_counter()
+    O.time_loop_1d(st, prm, 200, **kw)
+    one = (time.perf_counter() - t) / 200
+    n = max(200, int(seconds_target / max(one, 1e-7)))
+    t = time.perf_counter()
+    O.time_loop_1d(st, prm, n, **kw)
+    dt = time.perf_counter() - t
+    return {"value": L * n / dt / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": f"one L={L} film of the scan, FP64, {n} steps, {dt:.1f} s, "
+                      "unfused reference-order sweeps (oracle)"}
+
+
+def bench_1d(a):
+    """1D coalescence scan: a.films FP64 films of L = 1024 (scan parameters per
+    film) stepped together, one workgroup per film, whole films in LDS, K
+    steps in one launch (single1d_f64: one film, both loops)."""
+    import ctypes as C
+    import numpy as np
+    import swalbe as S
+    from swalbe import _lib
+    L = 1024
+    nf = 1 if a.workload == "single1d_f64" else a.films
+    films = [_scan_film(S, k, L, a.steps) for k in range(nf)]
+    systems = [f[0] for f in films]
+    states = [f[1] for f in films]
+    gammas = [f[2] for f in films]
+
+    def run(nsteps):
+        for sy in systems:
+            sy.param.Tmax = nsteps
+        w0 = time.perf_counter()
+        ms = S.ensemble_time_loop(systems, states, nsteps, γ=gammas, grad_gamma_sign=1,
+                                  materialize=False, threads=a.threads)
+        return ms / 1e3, time.perf_counter() - w0
+
+    run(a.warmup)
+    dt, wall = run(a.steps)
+    lups = nf * L * a.steps
+    glups = lups / dt / 1e9
+    line = {
+        "metric": "GLUPS of the 1D D1Q3 coalescence scan (films x L x steps / s)",
+        "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt * 1e3 / a.steps, 5),
+        "wall_GLUPS": round(lups / wall / 1e9, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (two droplets r=500, tanh gamma steps of the reference scan)",
+        "config": {"workload": f"{a.workload}: {nf} films x L={L}, FP64, gamma field + grad "
+                               "gamma force, half the films walled, tau=1, resident kernel",
+                   "films": nf, "L": L, "threads_per_film": a.threads or "auto",
+                   "parallelism": "single GPU, one workgroup per film"},
+        "roofline": {"bound": "fp64-valu", "achieved": round(glups * FLOP_PER_LUP_1D / 1e3, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(glups * FLOP_PER_LUP_1D / 1e3 / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None, "flop_per_lup": FLOP_PER_LUP_1D},
+    }
+    h = states[0].height
+    line["config"]["finite"] = bool(np.isfinite(h).all())
+    if a.workload == "single1d_f64":
+        # the same film with one launch per step
+        st2 = _scan_film(S, 0, L, a.steps)
+        o = _lib.StepOpts1D()
+        o.variant, o.grad_gamma, o.use_gamma_field = 1, 1, 1
+        st2[1].set("gamma", st2[2])
+        prm = st2[0].param.cparams()
+        ms = C.c_double()
+        _lib.check(_lib.lib().swalbe1d_time_loop_timed(st2[1].handle, C.byref(prm), a.steps,
+                                                       C.byref(o), C.byref(ms)), "1d loop")
+        line["per_step_loop_GLUPS"] = round(L * a.steps / (ms.value / 1e3) / 1e9, 4)
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_1d(L)
     print(json.dumps(line), flush=True)
 
 
@@ -246,6 +379,9 @@ def main():
     ap.add_argument("--tick", type=int, default=500, help="serve_poisson: steps per scheduler tick")
     ap.add_argument("--no-alone", action="store_true",
                     help="serve_poisson: skip the one-at-a-time baseline")
+    ap.add_argument("--films", type=int, default=2048, help="coalesce1d_scan: number of films")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="1D resident kernel: threads per film (0 = by L, 256, 512, 1024)")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--nontemporal", type=int, default=0)
@@ -264,6 +400,8 @@ def main():
         return bench_copy(a)
     if a.workload == "pattern4096_f32":
         return bench_pattern(a)
+    if a.workload in ("coalesce1d_scan", "single1d_f64"):
+        return bench_1d(a)
 
     import ctypes as C
     import numpy as np

@@ -1,6 +1,6 @@
 /*
  * swalbe_hip.h -- C ABI of libswalbe_hip, the MI355X (gfx950) implementation
- * of Swalbe.jl's 2D thin-film lattice Boltzmann time step.
+ * of Swalbe.jl's thin-film lattice Boltzmann time step (2D D2Q9 and 1D D1Q3).
  *
  * Plain C types only: handles, device/host pointers and sizes.  Every entry
  * point returns an int status (SWALBE_OK = 0) and never aborts; the text of
@@ -376,6 +376,9 @@ typedef struct swalbe1d_step_opts {
     double *mass_series;   /* host: Σh before each step t with t % tdump == 0 (:100-105) */
     int64_t snap_every;    /* snapshot!(fluid, h, t; dumping) (src/measures.jl:99-105) */
     double *snap_series;   /* host [nsnap * L], in step order, or NULL */
+    int32_t threads;       /* resident kernel: threads per film, 0 (by L), 256, 512 or 1024;
+                              an ensemble uses opts[0].threads */
+    int32_t pad_;
 } swalbe1d_step_opts;
 int swalbe1d_time_loop(swalbe_film1d *st, const swalbe_params *p, int64_t nsteps,
                        const swalbe1d_step_opts *opts);
@@ -391,6 +394,10 @@ int swalbe1d_time_loop_timed(swalbe_film1d *st, const swalbe_params *p, int64_t 
 int swalbe1d_ensemble_time_loop(swalbe_film1d *const *films, int32_t nfilms,
                                 const swalbe_params *params, int64_t nsteps,
                                 const swalbe1d_step_opts *opts);
+/* Same, reporting the device time of the launch (HIP events). */
+int swalbe1d_ensemble_time_loop_timed(swalbe_film1d *const *films, int32_t nfilms,
+                                      const swalbe_params *params, int64_t nsteps,
+                                      const swalbe1d_step_opts *opts, double *ms_out);
 /* Largest L that runs resident (whole film in LDS) on the current device. */
 int swalbe1d_resident_max_length(int32_t dtype, int32_t tau_is_one, int32_t *L);
 /* ∇f!(output::Vector, f, dgrad[, a]) (src/differences.jl:208-228) and

@@ -574,3 +574,49 @@ def time_loop_1d(state: State1D, prm: Params, nsteps: int, *, theta_field=None, 
     cs = state.cstruct()
     L.orc1d_time_loop(C.byref(cs), C.byref(prm), nsteps, C.byref(o))
     return out
+
+
+def time_loop_1d_f32(h, prm: Params, nsteps: int, gamma_field=None, grad_gamma=None,
+                     grad_gamma_sign=0, dtype=np.float32):
+    """The 1D loop (src/simulate.jl:98-116, run_gamma's force :548) in numpy
+    with every operation rounded to `dtype` in the library's order (periodic,
+    τ = 1, g = 0, no noise): float32 is the reference for FP32 states, and
+    float64 reproduces the C oracle to the bit (which pins this restatement).
+    Returns (h, u) after nsteps from rest.  κ is formed in FP64 and rounded
+    once, as the library does."""
+    f32 = dtype
+    assert prm.tau == 1.0 and prm.g == 0.0
+    L = len(h)
+    h = np.asarray(h, dtype=f32).copy()
+    u = np.zeros(L, dtype=f32)
+    kap = f32((((1.0 - cospi(prm.theta)) * float(prm.n - 1)) * float(prm.m - 1)) /
+              (float(prm.n - prm.m) * prm.hmin))
+    gam = np.full(L, f32(prm.gamma)) if gamma_field is None else np.asarray(gamma_field, dtype=f32)
+    gg = None if grad_gamma is None else np.asarray(grad_gamma, dtype=f32)
+    hmin, hcrit = f32(prm.hmin), f32(prm.hcrit)
+    c6mu, c6d, c3dd = f32(6.0 * prm.mu), f32(6.0 * prm.delta), f32(3.0 * (prm.delta * prm.delta))
+    zero, half, one, two = f32(0), f32(0.5), f32(1), f32(2)
+    for _ in range(nsteps):
+        hm, hp = np.roll(h, 1), np.roll(h, -1)
+        a = hmin / (h + hcrit)
+        pn = np.full(L, one)
+        for _k in range(prm.n):
+            pn = pn * a
+        pm_ = np.full(L, one)
+        for _k in range(prm.m):
+            pm_ = pm_ * a
+        p = -gam * (kap * (pn - pm_)) - gam * ((hm - two * h) + hp)
+        hgp = (h * f32(-0.5)) * (np.roll(p, 1) - np.roll(p, -1))
+        F = -hgp - (c6mu * h * u) / ((two * (h * h) + c6d * h) + c3dd)
+        if gg is not None:
+            F = F - gg if grad_gamma_sign < 0 else F + gg
+        uu = u * u
+        e0 = h * ((one - zero * h) - uu)
+        e1 = h * ((zero * h + half * u) + half * uu)
+        e2 = h * ((zero * h - half * u) + half * uu)
+        f0 = zero * zero + one * e0
+        f1 = np.roll((zero * zero + one * e1) + half * F, 1)
+        f2 = np.roll((zero * zero + one * e2) - half * F, -1)
+        h = (f0 + f1) + f2
+        u = (f1 - f2) / h
+    return h.astype(np.float64), u.astype(np.float64)

@@ -27,7 +27,7 @@ using swalbe::set_error;
 namespace {
 
 constexpr int kNT1 = 256;  // output nodes per workgroup of k1_step and the per-op kernels
-constexpr int kRT = 256;   // threads per film of k1_resident
+constexpr int kRTMax = 1024;  // most threads per film of k1_resident
 constexpr int64_t kResidentStaticLds = 256;  // k1_resident's static LDS (reduction slots)
 
 inline int blocks1d(int64_t n) {
@@ -278,6 +278,7 @@ struct Film1 {
   double wet_thresh;
   uint64_t seed;
   int64_t t0, tdump, snap_every;
+  int64_t mass_phase, snap_phase;  // steps before the first mass / snapshot step
   int L, n, m, thermal, body, read_ft, gg_sign;
 };
 
@@ -387,13 +388,18 @@ __global__ __launch_bounds__(kNT1) void k1_step(Film1 S, uint64_t step,
 
 // Whole films in LDS: workgroup b steps film table[b] through nsteps steps.
 // LDS: h, u, h', u', p, F (and f_t, f_t' when τ ≠ 1), L words each (3L for f).
-template <typename T>
-__global__ __launch_bounds__(kRT) void k1_resident(const Film1* __restrict__ table,
-                                                   int64_t nsteps) {
+// Only what every step uses is held in registers; the series pointers,
+// last-step outputs and noise constants are read from the table where they
+// are used (the table is not restrict: the loop's global stores keep those
+// loads in place), and the mass / snapshot schedule runs on countdowns.
+template <typename T, int kRT>
+__global__ __launch_bounds__(kRT) __attribute__((amdgpu_waves_per_eu(kRT == 512 ? 5 : 1)))
+void k1_resident(const Film1* table, int64_t nsteps) {
   extern __shared__ __align__(16) unsigned char smem1[];
   __shared__ double red[4][kRT / 64];
-  const Film1 S = table[blockIdx.x];
-  const int L = S.L, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const Film1* F1 = table + blockIdx.x;
+  const int L = F1->L, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int read_ft = F1->read_ft;
   T* h = (T*)smem1;
   T* u = h + L;
   T* hn = u + L;
@@ -402,37 +408,48 @@ __global__ __launch_bounds__(kRT) void k1_resident(const Film1* __restrict__ tab
   T* Fo = p + L;
   T* ft = Fo + L;
   T* ftn = ft + 3 * L;
-  const T* h_in = (const T*)S.h_in;
-  const T* u_in = (const T*)S.u_in;
-  for (int x = tid; x < L; x += kRT) {
-    h[x] = h_in[x];
-    u[x] = u_in[x];
+  {
+    const T* h_in = (const T*)F1->h_in;
+    const T* u_in = (const T*)F1->u_in;
+    for (int x = tid; x < L; x += kRT) {
+      h[x] = h_in[x];
+      u[x] = u_in[x];
+    }
+    if (read_ft) {
+      const T* ft_in = (const T*)F1->ft;
+      for (int i = tid; i < 3 * L; i += kRT) ft[i] = ft_in[i];
+    }
   }
-  if (S.read_ft)
-    for (int i = tid; i < 3 * L; i += kRT) ft[i] = ((const T*)S.ft)[i];
   __syncthreads();
-  const T gam_s = (T)S.gamma, kap_s = (T)S.kap_s, hmin = (T)S.hmin, hcrit = (T)S.hcrit;
-  const T c6mu = (T)S.c6mu, c6d = (T)S.c6d, c3dd = (T)S.c3dd, kb26 = (T)S.kb26, delta = (T)S.delta;
-  const T g = (T)S.g, omeg = (T)S.omeg, itau = (T)S.itau, alpha = (T)S.alpha, fac = (T)S.fac;
-  const T* kap_f = (const T*)S.kap;
-  const T* gam_f = (const T*)S.gam;
-  const T* gg = (const T*)S.gg;
-  const T* b1 = (const T*)S.b1;
-  const T* b2 = (const T*)S.b2;
-  const T* inter = (const T*)S.interior;
+  const int n = F1->n, m = F1->m, gg_sign = F1->gg_sign, thermal = F1->thermal, body = F1->body;
+  const T gam_s = (T)F1->gamma, kap_s = (T)F1->kap_s, hmin = (T)F1->hmin, hcrit = (T)F1->hcrit;
+  const T c6mu = (T)F1->c6mu, c6d = (T)F1->c6d, c3dd = (T)F1->c3dd;
+  const T g = (T)F1->g, omeg = (T)F1->omeg, itau = (T)F1->itau;
+  const T* kap_f = (const T*)F1->kap;
+  const T* gam_f = (const T*)F1->gam;
+  const T* gg = (const T*)F1->gg;
+  const T* b1 = (const T*)F1->b1;
+  const T* b2 = (const T*)F1->b2;
+  const bool series = F1->dh || F1->wet || F1->mass;
+  // countdowns to the next mass (t % tdump == 0) and snapshot step
+  int64_t cm = F1->mass ? F1->mass_phase : -1, cs = F1->snap ? F1->snap_phase : -1;
   int64_t km = 0, ks = 0;
   for (int64_t it = 1; it <= nsteps; ++it) {
-    const int64_t t = S.t0 + it;
+    // the table pointer is opaque from here on: cold fields are reloaded
+    // (scalar-cache hits) instead of occupying scalar registers all loop
+    asm volatile("" : "+s"(F1));
     const bool last = it == nsteps;
-    const bool do_mass = S.mass && S.tdump > 0 && t % S.tdump == 0;
-    if (do_mass || S.dh || S.wet) {  // series over the step's input height
+    const bool do_mass = cm == 0;
+    if (series) {  // series over the step's input height
+      const T* inter = (const T*)F1->interior;
+      const double wthr = F1->wet_thresh;
       double s = 0.0, mx = -__builtin_inf(), mn = __builtin_inf(), c = 0.0;
       for (int x = tid; x < L; x += kRT) {
         const double hv = (double)h[x];
         s += inter ? (double)(h[x] - inter[x] * h[x]) : hv;
         mx = hv > mx ? hv : mx;
         mn = hv < mn ? hv : mn;
-        c += hv > S.wet_thresh ? 1.0 : 0.0;
+        c += hv > wthr ? 1.0 : 0.0;
       }
       s = wave_sum(s);
       mx = wave_max(mx);
@@ -453,16 +470,20 @@ __global__ __launch_bounds__(kRT) void k1_resident(const Film1* __restrict__ tab
           n0 = red[2][w] < n0 ? red[2][w] : n0;
           c0 += red[3][w];
         }
-        if (do_mass) S.mass[km] = s0;
-        if (S.dh) S.dh[it - 1] = m0 - n0;
-        if (S.wet) S.wet[it - 1] = c0;
+        if (do_mass) F1->mass[km] = s0;
+        if (F1->dh) F1->dh[it - 1] = m0 - n0;
+        if (F1->wet) F1->wet[it - 1] = c0;
       }
-      km += do_mass ? 1 : 0;
     }
+    if (do_mass) {
+      ++km;
+      cm = F1->tdump;
+    }
+    if (cm > 0) --cm;
     for (int x = tid; x < L; x += kRT) {
       const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
       p[x] = pressure_node(h[xm], h[x], h[xp], gam_f ? gam_f[x] : gam_s,
-                           kap_f ? kap_f[x] : kap_s, S.n, S.m, hmin, hcrit, 0);
+                           kap_f ? kap_f[x] : kap_s, n, m, hmin, hcrit, 0);
     }
     __syncthreads();
     for (int x = tid; x < L; x += kRT) {
@@ -470,9 +491,11 @@ __global__ __launch_bounds__(kRT) void k1_resident(const Film1* __restrict__ tab
       const T hh = h[x];
       const T hgp = (hh * T(-0.5)) * (p[xm] - p[xp]);
       T F = -hgp - slip_node(hh, u[x], c6mu, c6d, c3dd);
-      if (S.thermal) F = F - noise_node<T>(S.seed, (uint64_t)t, x) * noise_amp(hh, kb26, delta);
-      if (gg) F = S.gg_sign < 0 ? F - gg[x] : F + gg[x];
-      if (S.body) F = F + (hh * alpha) * fac;
+      if (thermal)
+        F = F - noise_node<T>(F1->seed, (uint64_t)(F1->t0 + it), x) *
+                    noise_amp(hh, (T)F1->kb26, (T)F1->delta);
+      if (gg) F = gg_sign < 0 ? F - gg[x] : F + gg[x];
+      if (body) F = F + (hh * (T)F1->alpha) * (T)F1->fac;
       Fo[x] = F;
     }
     __syncthreads();
@@ -482,7 +505,7 @@ __global__ __launch_bounds__(kRT) void k1_resident(const Film1* __restrict__ tab
         T e0, e1, e2;
         feq_node(h[y], u[y], g, e0, e1, e2);
         const T fe = k == 0 ? e0 : (k == 1 ? e1 : e2);
-        const T tk = S.read_ft ? ft[k * L + y] : T(0);
+        const T tk = read_ft ? ft[k * L + y] : T(0);
         T v = omeg * tk + itau * fe;
         if (k == 1) v = v + T(0.5) * Fo[y];
         if (k == 2) v = v - T(0.5) * Fo[y];
@@ -498,21 +521,21 @@ __global__ __launch_bounds__(kRT) void k1_resident(const Film1* __restrict__ tab
       const T uv = (f1 - f2) / hv;
       hn[x] = hv;
       un[x] = uv;
-      if (S.read_ft) {
+      if (read_ft) {
         ftn[x] = f0;
         ftn[L + x] = f1;
         ftn[2 * L + x] = f2;
       }
       if (last) {
-        T* fout = (T*)S.f_out;
+        T* fout = (T*)F1->f_out;
         fout[x] = f0;
         fout[L + x] = f1;
         fout[2 * L + x] = f2;
-        ((T*)S.h_out)[x] = hv;
-        ((T*)S.u_out)[x] = uv;
-        ((T*)S.p_out)[x] = p[x];
-        ((T*)S.h_last)[x] = h[x];
-        ((T*)S.u_last)[x] = u[x];
+        ((T*)F1->h_out)[x] = hv;
+        ((T*)F1->u_out)[x] = uv;
+        ((T*)F1->p_out)[x] = p[x];
+        ((T*)F1->h_last)[x] = h[x];
+        ((T*)F1->u_last)[x] = u[x];
       }
     }
     __syncthreads();
@@ -525,10 +548,13 @@ __global__ __launch_bounds__(kRT) void k1_resident(const Film1* __restrict__ tab
     tmp = ft;
     ft = ftn;
     ftn = tmp;
-    if (S.snap && t % S.snap_every == 0) {  // snapshot!(…, h, t) after the step
-      for (int x = tid; x < L; x += kRT) S.snap[ks * L + x] = (double)h[x];
+    if (cs == 0) {  // snapshot!(…, h, t) after the step
+      double* snap = F1->snap + ks * L;
+      for (int x = tid; x < L; x += kRT) snap[x] = (double)h[x];
       ++ks;
+      cs = F1->snap_every;
     }
+    if (cs > 0) --cs;
   }
 }
 
@@ -725,6 +751,8 @@ int check_opts(swalbe_film1d* s, const swalbe_params* p, const swalbe1d_step_opt
   if (o->bound && !s->have_border) return bad1("swalbe1d_time_loop: no border masks uploaded");
   if (o->variant < 0 || o->variant > 2) return bad1("swalbe1d_time_loop: variant is 0, 1 or 2");
   if (o->snap_every < 0) return bad1("swalbe1d_time_loop: snap_every < 0");
+  if (o->threads != 0 && o->threads != 256 && o->threads != 512 && o->threads != 1024)
+    return bad1("swalbe1d_time_loop: threads is 0, 256, 512 or 1024");
   return SWALBE_OK;
 }
 
@@ -776,6 +804,9 @@ Film1 describe(swalbe_film1d* s, const swalbe_params* p, const swalbe1d_step_opt
   S.t0 = o->t0;
   S.tdump = p->tdump;
   S.snap_every = o->snap_every;
+  // step it (1-based) is a mass step when (t0 + it) % tdump == 0: the first is it = phase + 1
+  S.mass_phase = p->tdump > 0 ? (p->tdump - (o->t0 + 1) % p->tdump) % p->tdump : -1;
+  S.snap_phase = o->snap_every > 0 ? (o->snap_every - (o->t0 + 1) % o->snap_every) % o->snap_every : -1;
   S.wet_thresh = o->wet_thresh;
   S.kap = o->use_theta_field ? s->kap : nullptr;
   S.gam = o->use_gamma_field ? s->gamma : nullptr;
@@ -1157,11 +1188,39 @@ void finish(swalbe_film1d* s, int64_t nsteps) {
   s->cur = 1 - s->cur;
 }
 
+template <typename T, int NT>
+hipError_t launch_resident_t(const Film1* d_table, int nfilms, int64_t nsteps, int64_t lds,
+                             hipStream_t st) {
+  if (lds + kResidentStaticLds > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void*)k1_resident<T, NT>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k1_resident<T, NT>), dim3(nfilms), dim3(NT), (size_t)lds, st, d_table,
+                     nsteps);
+  return hipGetLastError();
+}
+
+// threads per film (0 = automatic).  Measured on MI355X, FP64, L = 1024:
+// with at least one film per CU, 512 threads (5 waves per SIMD fit by
+// registers, 2 films per CU) step the ensemble fastest; a few films are
+// latency bound and run fastest at one node per thread (1024).
+int resident_threads(int requested, int maxL, int nfilms, int device) {
+  if (requested == 256 || requested == 512 || requested == 1024) return requested;
+  if (maxL <= 256) return 256;
+  int cus = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    cus = 256;
+  if (maxL > 512 && nfilms < cus) return 1024;
+  return 512;
+}
+
 template <typename T>
-hipError_t set_resident_lds(int64_t bytes) {
-  if (bytes + kResidentStaticLds <= 65536) return hipSuccess;
-  return hipFuncSetAttribute((const void*)k1_resident<T>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+hipError_t launch_resident(const Film1* d_table, int nfilms, int64_t nsteps, int64_t lds, int nt,
+                           hipStream_t st) {
+  if (nt == 256) return launch_resident_t<T, 256>(d_table, nfilms, nsteps, lds, st);
+  if (nt == 512) return launch_resident_t<T, 512>(d_table, nfilms, nsteps, lds, st);
+  return launch_resident_t<T, 1024>(d_table, nfilms, nsteps, lds, st);
 }
 
 // one launch per step (any L): the moments and distributions ping-pong
@@ -1259,10 +1318,12 @@ int loop_per_step(swalbe_film1d* s, const swalbe_params* p, int64_t nsteps,
 
 // k1_resident over a list of films (one dtype, one device), launched on st
 int loop_resident(swalbe_film1d* const* films, int nfilms, const swalbe_params* params,
-                  int64_t nsteps, const swalbe1d_step_opts* const* opts, hipStream_t st) {
+                  int64_t nsteps, const swalbe1d_step_opts* const* opts, hipStream_t st,
+                  double* ms_out) {
   std::vector<Film1> table(nfilms);
   std::vector<Series1> series(nfilms);
   int64_t lds = 0;
+  int maxL = 0;
   int rc = SWALBE_OK;
   for (int i = 0; i < nfilms && rc == SWALBE_OK; ++i) {
     swalbe_film1d* s = films[i];
@@ -1278,6 +1339,7 @@ int loop_resident(swalbe_film1d* const* films, int nfilms, const swalbe_params* 
     table[i].mass = series[i].mass;
     table[i].snap = series[i].snap;
     lds = std::max(lds, resident_bytes(s->L, s->esz, table[i].read_ft));
+    maxL = std::max(maxL, s->L);
   }
   Film1* d_table = nullptr;
   hipError_t e = hipSuccess;
@@ -1285,21 +1347,25 @@ int loop_resident(swalbe_film1d* const* films, int nfilms, const swalbe_params* 
     e = hipMalloc(&d_table, sizeof(Film1) * nfilms);
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_table, table.data(), sizeof(Film1) * nfilms, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-      if (films[0]->esz == 8) {
-        e = set_resident_lds<double>(lds);
-        if (e == hipSuccess)
-          hipLaunchKernelGGL(k1_resident<double>, dim3(nfilms), dim3(kRT), (size_t)lds, st,
-                             d_table, nsteps);
-      } else {
-        e = set_resident_lds<float>(lds);
-        if (e == hipSuccess)
-          hipLaunchKernelGGL(k1_resident<float>, dim3(nfilms), dim3(kRT), (size_t)lds, st,
-                             d_table, nsteps);
-      }
+    const int nt = resident_threads(opts[0]->threads, maxL, nfilms, films[0]->device);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (e == hipSuccess && ms_out) {
+      e = hipEventCreate(&ev[0]);
+      if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+      if (e == hipSuccess) e = hipEventRecord(ev[0], st);
     }
-    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess)
+      e = films[0]->esz == 8 ? launch_resident<double>(d_table, nfilms, nsteps, lds, nt, st)
+                             : launch_resident<float>(d_table, nfilms, nsteps, lds, nt, st);
+    if (e == hipSuccess && ms_out) e = hipEventRecord(ev[1], st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && ms_out) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+      *ms_out = ms;
+    }
+    for (hipEvent_t v : ev)
+      if (v) (void)hipEventDestroy(v);
     for (int i = 0; i < nfilms && e == hipSuccess; ++i) {
       finish(films[i], nsteps);
       e = series[i].fetch(films[i], opts[i], st);
@@ -1328,7 +1394,7 @@ int swalbe1d_time_loop(swalbe_film1d* s, const swalbe_params* p, int64_t nsteps,
   const bool resident = o->variant == 2 || (o->variant == 0 && fits_resident(s, p));
   int rc;
   if (resident) {
-    rc = loop_resident(&s, 1, p, nsteps, &o, s->stream);
+    rc = loop_resident(&s, 1, p, nsteps, &o, s->stream, nullptr);
   } else {
     if (o->use_theta_field)
       if ((rc = refresh_kappa(s, p->n, p->m, p->hmin))) return rc;
@@ -1342,9 +1408,9 @@ int swalbe1d_time_loop(swalbe_film1d* s, const swalbe_params* p, int64_t nsteps,
   return rc;
 }
 
-int swalbe1d_ensemble_time_loop(swalbe_film1d* const* films, int32_t nfilms,
-                                const swalbe_params* params, int64_t nsteps,
-                                const swalbe1d_step_opts* opts) {
+int swalbe1d_ensemble_time_loop_timed(swalbe_film1d* const* films, int32_t nfilms,
+                                      const swalbe_params* params, int64_t nsteps,
+                                      const swalbe1d_step_opts* opts, double* ms_out) {
   if (!films || nfilms < 1 || !params || nsteps < 0)
     return bad1("swalbe1d_ensemble_time_loop: bad arguments");
   swalbe1d_step_opts zero{};
@@ -1367,7 +1433,7 @@ int swalbe1d_ensemble_time_loop(swalbe_film1d* const* films, int32_t nfilms,
       return bad1("swalbe1d_ensemble_time_loop: a film is listed twice");
   }
   (void)hipSetDevice(films[0]->device);
-  int rc = loop_resident(films, nfilms, params, nsteps, po.data(), films[0]->stream);
+  int rc = loop_resident(films, nfilms, params, nsteps, po.data(), films[0]->stream, ms_out);
   for (int i = 0; i < nfilms && rc == SWALBE_OK; ++i)
     if (po[i]->materialize && nsteps > 0) {
       materialize(films[i], &params[i], po[i], nsteps);
@@ -1375,6 +1441,12 @@ int swalbe1d_ensemble_time_loop(swalbe_film1d* const* films, int32_t nfilms,
         rc = bad1("swalbe1d_ensemble_time_loop: materialize failed");
     }
   return rc;
+}
+
+int swalbe1d_ensemble_time_loop(swalbe_film1d* const* films, int32_t nfilms,
+                                const swalbe_params* params, int64_t nsteps,
+                                const swalbe1d_step_opts* opts) {
+  return swalbe1d_ensemble_time_loop_timed(films, nfilms, params, nsteps, opts, nullptr);
 }
 
 int swalbe1d_time_loop_timed(swalbe_film1d* s, const swalbe_params* p, int64_t nsteps,

@@ -74,6 +74,7 @@ class StepOpts1D(C.Structure):
         ("wet_thresh", C.c_double), ("dh_series", C.POINTER(C.c_double)),
         ("wet_series", C.POINTER(C.c_int64)), ("mass_series", C.POINTER(C.c_double)),
         ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
+        ("threads", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -206,6 +207,8 @@ def lib():
          [_V, C.POINTER(Params), C.c_int64, C.POINTER(StepOpts1D), C.POINTER(_D)])
     _sig(L, "swalbe1d_ensemble_time_loop",
          [C.POINTER(_V), _I, C.POINTER(Params), C.c_int64, C.POINTER(StepOpts1D)])
+    _sig(L, "swalbe1d_ensemble_time_loop_timed",
+         [C.POINTER(_V), _I, C.POINTER(Params), C.c_int64, C.POINTER(StepOpts1D), C.POINTER(_D)])
     _sig(L, "swalbe1d_resident_max_length", [_I, _I, C.POINTER(_I)])
     _sig(L, "swalbe1d_grad_ptr", [_I, _V, _V, _V, _I, _V])
     _sig(L, "swalbe1d_laplacian_ptr", [_I, _V, _V, _I, _V])

@@ -180,9 +180,15 @@ def Sys(sysc: SysConst_1D, T=np.float64, kind: str = "simple") -> HipState1D:
 
 
 def _upload_walls(state: HipState1D, sys: SysConstWithBound_1D) -> None:
+    """Uploads the wall masks when they differ from the ones the state holds."""
+    key = b"".join(np.ascontiguousarray(a, dtype=np.float64).tobytes()
+                   for a in (sys.border[0], sys.border[1], sys.interior))
+    if getattr(state, "_walls", None) == key:
+        return
     state.set("border1", sys.border[0])
     state.set("border2", sys.border[1])
     state.set("interior", sys.interior)
+    object.__setattr__(state, "_walls", key)
 
 
 # ---------------------------------------------------------------- ops
@@ -401,11 +407,11 @@ def _time_loop_callback(sys, state, f: Callable, measure, verbose, thermal_on, s
 
 def ensemble_time_loop(systems: Sequence, states: Sequence[HipState1D], nsteps: Optional[int] = None,
                        thermal=False, seed=0, materialize: bool = True, γ=None,
-                       grad_gamma_sign=0, θ=None) -> None:
+                       grad_gamma_sign=0, θ=None, threads: int = 0) -> float:
     """Steps many 1D films in one launch (one workgroup per film), each with
     its own SysConst_1D.  Per-film options may be given as sequences
     (thermal, seed, γ, θ, grad_gamma_sign); each film ends exactly as its own
-    time_loop(..., variant=2) would."""
+    time_loop(..., variant=2) would.  Returns the launch's device time in ms."""
     n = len(states)
     if len(systems) != n:
         raise ValueError("one system per state")
@@ -430,11 +436,15 @@ def ensemble_time_loop(systems: Sequence, states: Sequence[HipState1D], nsteps: 
                 o.use_theta_field = 1
             else:
                 prms[i].theta = float(th)
+        o.threads = int(threads)
         opts[i] = o
     handles = (C.c_void_p * n)(*[st.handle.value for st in states])
-    check(lib().swalbe1d_ensemble_time_loop(handles, n, prms, steps, opts), "ensemble_time_loop")
+    ms = C.c_double(0.0)
+    check(lib().swalbe1d_ensemble_time_loop_timed(handles, n, prms, steps, opts, C.byref(ms)),
+          "ensemble_time_loop")
     for st in states:
         st.t += steps
+    return ms.value
 
 
 def resident_max_length(T=np.float64, tau_is_one: bool = True) -> int:

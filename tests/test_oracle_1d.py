@@ -176,3 +176,21 @@ def test_grad_gamma_force_sign_drives_the_flow():
     assert np.allclose(vel[-1], -vel[1], rtol=2e-2, atol=1e-15)
     far = np.r_[4:j - 4, j + 5:L - 4]
     assert np.all(vel[-1][far] == 0.0)
+
+
+def test_numpy_restatement_in_float64_is_the_c_oracle():
+    # the dtype-generic restatement (the FP32 reference) run in float64 equals
+    # the C oracle bit for bit, with and without the γ terms
+    L, n = 128, 60
+    x = np.arange(L)
+    h = 1.0 + 0.2 * np.sin(2 * np.pi * 3 * x / L)
+    prm = O.params(tdump=1000)
+    gam = 0.01 + 0.002 * np.sin(2 * np.pi * x / L)
+    gg = O.grad_gamma_1d(gam, form=0)
+    for kw in (dict(), dict(gamma_field=gam, grad_gamma=gg, grad_gamma_sign=-1)):
+        st = O.State1D(L)
+        st.height[...] = h
+        st.feq[...] = O.equilibrium_1d(h, np.zeros(L), 0.0)
+        O.time_loop_1d(st, prm, n, **kw)
+        h64, u64 = O.time_loop_1d_f32(h, prm, n, dtype=np.float64, **kw)
+        assert np.array_equal(h64, st.height) and np.array_equal(u64, st.vel)
