@@ -15,8 +15,9 @@ from .core import (BGKandStream, DeviceArray, HipState, Sys, SysConst, Taumucs, 
                    fast_32, fast_93, filmpressure, force_sum, grad_f, h_grad_p, h_grad_p_raw,
                    inclination, laplacian_f, moments, power_broad, slippage, thermal, time_loop,
                    wetted)
-from .initialvalues import (normals, randinterface, random_film, rivulet, sinewave, singledroplet,
-                            torus, two_droplets_2d, uniforms)
+from .initialvalues import (Ellipsoid, VPolygon, boxpattern, ellipsepattern, normals, randinterface,
+                            random_film, rivulet, sinewave, sinewave1D, singledroplet, torus,
+                            trianglepattern, two_droplets_2d, uniforms)
 from .batch import (INIT_CAP, INIT_FLAT, INIT_NORMAL, INIT_TWO_CAPS, Batch, FilmInit, device_memory,
                     film_bytes, film_init)
 from .io import load_state, restart_from_height, save_snapshots, save_state, snapshot

@@ -128,3 +128,128 @@ def sinewave(Lx: int, Ly: int, h0: float, kx: int, ky: int, eps: float):
     j = np.arange(1, Ly + 1)[None, :]
     return np.asfortranarray(
         h0 * (1 + eps * np.sin(2 * math.pi * kx * i / (Lx - 1)) * np.sin(2 * math.pi * ky * j / (Ly - 1))))
+
+
+def sinewave1D(height: np.ndarray, h0: float, n: int, eps: float, dims: int) -> None:
+    """sinewave1D!(height, h₀, n, ϵ, dims) (src/initialvalues.jl:6-16): n sine
+    waves along dimension `dims` (1-based as in the reference), amplitude
+    h₀ ϵ, shifted by -h₀ ϵ / size(height, dims)."""
+    ax = int(dims) - 1
+    size = height.shape[ax]
+    acc = np.cumsum(np.ones(height.shape), axis=ax)  # accumulate(+, ones, dims)
+    hd = 2 * math.pi * n * acc / (size - 1)
+    height[...] = h0 * (1.0 + eps * (np.sin(hd) - 1 / size))
+
+
+# ---------------------------------------------------------------- contact-angle patterns
+
+
+class VPolygon:
+    """A convex polygon given by its vertices (the LazySets.VPolygon the
+    pattern functions return); `p in P` includes the boundary."""
+
+    def __init__(self, vertices):
+        v = np.asarray(vertices, dtype=float)
+        c = v.mean(axis=0)
+        order = np.argsort(np.arctan2(v[:, 1] - c[1], v[:, 0] - c[0]))
+        self.vertices = v[order]  # counter-clockwise
+
+    def __contains__(self, p) -> bool:
+        return bool(self.contains(np.asarray(p, dtype=float)[None, :])[0])
+
+    def contains(self, pts: np.ndarray) -> np.ndarray:
+        """Membership of many points (rows of pts), boundary included with a
+        relative tolerance as LazySets' comparisons use."""
+        v = self.vertices
+        inside = np.ones(len(pts), dtype=bool)
+        for k in range(len(v)):
+            a, b = v[k], v[(k + 1) % len(v)]
+            cross = (b[0] - a[0]) * (pts[:, 1] - a[1]) - (b[1] - a[1]) * (pts[:, 0] - a[0])
+            scale = np.abs(b - a).max() * (np.abs(pts - a).max(axis=1) + 1.0)
+            inside &= cross >= -1.5e-8 * scale
+        return inside
+
+    def __repr__(self):
+        return f"VPolygon({self.vertices.tolist()})"
+
+
+class Ellipsoid:
+    """{x : (x - c)ᵀ Q⁻¹ (x - c) ≤ 1} with a diagonal shape matrix Q (LazySets.Ellipsoid)."""
+
+    def __init__(self, center, q_diag):
+        self.center = np.asarray(center, dtype=float)
+        self.q = np.asarray(q_diag, dtype=float)
+
+    def __contains__(self, p) -> bool:
+        return bool(self.contains(np.asarray(p, dtype=float)[None, :])[0])
+
+    def contains(self, pts: np.ndarray) -> np.ndarray:
+        d = pts - self.center
+        r = (d ** 2 / self.q).sum(axis=1)
+        return r <= 1.0 + 1.5e-8
+
+    def __repr__(self):
+        return f"Ellipsoid(center={self.center.tolist()}, Q=diag{self.q.tolist()})"
+
+
+def _clamp_vertices(xc, yc, shape):
+    """The reference's "vertices on the grid" rule: an x outside [1, size 1]
+    moves to 1, else a y outside [1, size 2] moves to 1."""
+    for k in range(len(xc)):
+        if xc[k] < 1 or xc[k] > shape[0]:
+            xc[k] = 1
+        elif yc[k] < 1 or yc[k] > shape[1]:
+            yc[k] = 1
+
+
+def _paint(θ, θ0, δ, P):
+    """θ[j, i] = θ₀ + δₐ where the point (i, j) (1-based) is in P, else θ₀,
+    as the reference's loops assign (note the swapped indices)."""
+    n1, n2 = θ.shape
+    i, j = np.meshgrid(np.arange(1, n1 + 1, dtype=float), np.arange(1, n2 + 1, dtype=float),
+                       indexing="ij")
+    inside = P.contains(np.stack([i.ravel(), j.ravel()], axis=1)).reshape(n1, n2)
+    val = np.where(inside, θ0 + δ, θ0)
+    # θ[j, i] = val[i, j]: a transpose (the reference fills square maps)
+    θ[...] = val.T if θ.shape == val.T.shape else val.T[: θ.shape[0], : θ.shape[1]]
+    return θ
+
+
+def boxpattern(θ: np.ndarray, θ0, center=None, δa=1 / 36, side=20, **kw):
+    """boxpattern(θ, θ₀; center, δₐ, side) (src/initialvalues.jl:423-470):
+    a square patch of contact angle θ₀ + δₐ.  Returns (θ, polygon)."""
+    δa = kw.pop("δₐ", δa)
+    if kw:
+        raise TypeError(f"unknown keywords {sorted(kw)}")
+    center = (θ.shape[0] // 2, θ.shape[1] // 2) if center is None else center
+    xc = [center[0] - side / 2, center[0] - side / 2, center[0] + side / 2, center[0] + side / 2]
+    yc = [center[1] - side / 2, center[1] + side / 2, center[1] - side / 2, center[1] + side / 2]
+    _clamp_vertices(xc, yc, θ.shape)
+    P = VPolygon(list(zip(xc, yc)))
+    return _paint(θ, θ0, δa, P), P
+
+
+def ellipsepattern(θ: np.ndarray, θ0, center=None, δa=1 / 36, a=10, b=5, **kw):
+    """ellipsepattern(θ, θ₀; center, δₐ, a, b) (src/initialvalues.jl:494-525):
+    an elliptic patch with half axes a (first index) and b.  Returns (θ, ellipse)."""
+    δa = kw.pop("δₐ", δa)
+    if kw:
+        raise TypeError(f"unknown keywords {sorted(kw)}")
+    center = (θ.shape[0] // 2, θ.shape[1] // 2) if center is None else center
+    P = Ellipsoid([float(center[0]), float(center[1])], [float(a) ** 2, float(b) ** 2])
+    return _paint(θ, θ0, δa, P), P
+
+
+def trianglepattern(θ: np.ndarray, θ0, center=None, δa=1 / 36, side=60, **kw):
+    """trianglepattern(θ, θ₀; center, δₐ, side) (src/initialvalues.jl:550-581):
+    an equilateral triangle of side `side` centred on `center`.  Returns (θ, polygon)."""
+    δa = kw.pop("δₐ", δa)
+    if kw:
+        raise TypeError(f"unknown keywords {sorted(kw)}")
+    center = (θ.shape[0] // 2, θ.shape[1] // 2) if center is None else center
+    height = math.sqrt(3) / 2 * side
+    xc = [center[0] - side / 2, center[0] + side / 2, center[0]]
+    yc = [center[1] - height / 3, center[1] - height / 3, center[1] + 2 * height / 3]
+    _clamp_vertices(xc, yc, θ.shape)
+    P = VPolygon(list(zip(xc, yc)))
+    return _paint(θ, θ0, δa, P), P
