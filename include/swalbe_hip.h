@@ -168,6 +168,18 @@ typedef struct swalbe_step_opts {
      * 0 / NULL = off. */
     int64_t snap_every;
     double *snap_series;
+    /* measurement callbacks of time_loop(sys, state, f, measure) (src/simulate.jl:69-96)
+     * taken on the device on each step's input height, copied back after the loop:
+     * wetted!(area, maxheight, h, t) (src/measures.jl:6-11), fluid_dry!(fluid, dummy,
+     * h, t) (:23-28), surfacearea!(area_lv, red_energy, h, θ, …, t) (:126-168).
+     * NULL = off. */
+    double *maxh_series;    /* host [nsteps]: maximum(h) */
+    double *area_series;    /* host [nsteps]: Σ sqrt(|∇h|² + 1) over h > surf_thresh */
+    double *energy_series;  /* host [nsteps]: area - Σ cos πθ over h > surf_thresh */
+    uint8_t *fluid_series;  /* host [nsteps * Lx * Ly]: h > surf_thresh per step */
+    double surf_thresh;     /* htresh / hthresh of those (0 = 0.055) */
+    int32_t surf_theta_field; /* reduced energy from the θ field (else cos πθ of params) */
+    int32_t pad2_;
 } swalbe_step_opts;
 int swalbe_time_loop(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
                      const swalbe_step_opts *opts);
@@ -180,6 +192,15 @@ int swalbe_time_loop_timed(swalbe_state *st, const swalbe_params *p, int64_t nst
  * (:56) and count(h .> thresh) (src/measures.jl:13-17), in FP64.
  * out = {sum, max, min, count}.  Deterministic order. */
 int swalbe_reduce(swalbe_state *st, int32_t field, double thresh, double out[4]);
+
+/* ---- measurements of the height on the device (src/measures.jl):
+ * out = {count(h > thresh), maximum(h), Σ sqrt(|∇h|² + 1) over h > thresh
+ * (∇f_simple!, :175-192), that area minus Σ cos πθ over h > thresh} --
+ * wetted! (:6-17) and surfacearea! (:126-168); surface = 0 skips the last
+ * two.  θ is the scalar `theta` or, with use_theta_field, the θ field.
+ * host_mask (Lx*Ly bytes, or NULL) receives fluid_dry!'s mask (:23-28). */
+int swalbe_measure(swalbe_state *st, double thresh, int32_t use_theta_field, double theta,
+                   int32_t surface, double out[4], uint8_t *host_mask, size_t mask_bytes);
 
 /* ---- seeded initial height on the device: h = base + amp * U(0,1)
  * (dist 0) or base + amp * N(0,1) (dist 1), Philox keyed (seed, node);
@@ -436,6 +457,15 @@ int swalbe_thermal_ptr(int32_t dtype, void *kbtx, void *kbty, const void *h, dou
 int swalbe_inclination_ptr(int32_t dtype, void *Fx, void *Fy, const void *h, double ax,
                            double ay, double fac, int32_t Lx, int32_t Ly,
                            void *stream); /* src/forcing.jl:363-368 */
+/* surfacearea! / wetted! / fluid_dry! on a caller's height array (device
+ * pointers, plain values): out_dev = 4 doubles as swalbe_measure; mask
+ * (device, Lx*Ly bytes) may be NULL.  Synchronises `stream`. */
+int swalbe_measure_ptr(int32_t dtype, const void *h, const void *theta_field, double theta,
+                       int32_t Lx, int32_t Ly, double thresh, void *mask, double *out_dev,
+                       void *stream);
+/* ∇f_simple!(outx, outy, f, dgrad) -- src/measures.jl:175-192 */
+int swalbe_grad_simple_ptr(int32_t dtype, void *outx, void *outy, const void *f, int32_t Lx,
+                           int32_t Ly, void *stream);
 /* ---- host-side draws of the same counter-based streams (initial
  * conditions built on the host): node i of n gets the pair / uniform of
  * counter (seed, step, i, tag).  cospi as used for κ(θ). */

@@ -620,3 +620,31 @@ def time_loop_1d_f32(h, prm: Params, nsteps: int, gamma_field=None, grad_gamma=N
         h = (f0 + f1) + f2
         u = (f1 - f2) / h
     return h.astype(np.float64), u.astype(np.float64)
+
+
+def grad_simple(f):
+    """∇f_simple! (src/measures.jl:175-192) of an (Lx, Ly) array."""
+    f = _f(f)
+    Lx, Ly = f.shape
+    gx, gy = np.zeros((Lx, Ly), order="F"), np.zeros((Lx, Ly), order="F")
+    lib().orc_grad_simple(_p(gx), _p(gy), _p(f), Lx, Ly)
+    return gx, gy
+
+
+def measure(h, theta_field=None, theta=0.0, thresh=0.055):
+    """wetted! / fluid_dry! / surfacearea! of a height field: returns
+    ({count, max, area, energy}, mask)."""
+    h = _f(h)
+    Lx, Ly = h.shape
+    out = np.zeros(4)
+    mask = np.zeros((Lx, Ly), dtype=np.uint8, order="F")
+    tf = _f(theta_field, (Lx, Ly)) if theta_field is not None else None
+    lib().orc_measure(_p(h), _p(tf), C.c_double(theta), Lx, Ly, C.c_double(thresh), _p(out),
+                      mask.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    return {"count": out[0], "max": out[1], "area": out[2], "energy": out[3]}, mask
+
+
+def t0(hb=0.07, gamma=0.01, mu=1 / 6, theta=1 / 6):
+    out = np.zeros(2)
+    lib().orc_t0(C.c_double(hb), C.c_double(gamma), C.c_double(mu), C.c_double(theta), _p(out))
+    return out[0], out[1]

@@ -522,3 +522,60 @@ int orc_time_loop(orc_state *s, const orc_params *p, int64_t nsteps, const orc_l
     }
     return ORC_OK;
 }
+
+/* ∇f_simple!(outx, outy, f, dgrad): src/measures.jl:175-192, the 9-point
+ * gradient written with -1/3 and -1/12 weights in this order.
+ * Pinned: test/measures.jl:67-78 (exact Xgrad / Ygrad of reshape(1:25, 5, 5)). */
+void orc_grad_simple(double *outx, double *outy, const double *f, int Lx, int Ly) {
+    for (int y = 0; y < Ly; ++y)
+        for (int x = 0; x < Lx; ++x) {
+            const int xm = (x - 1 + Lx) % Lx, xp = (x + 1) % Lx;
+            const int ym = (y - 1 + Ly) % Ly, yp = (y + 1) % Ly;
+#define F_(a, b) f[(a) + (int64_t)Lx * (b)]
+            const double fip = F_(xm, y), fim = F_(xp, y), fjp = F_(x, ym), fjm = F_(x, yp);
+            const double fipjp = F_(xm, ym), fimjp = F_(xp, ym), fimjm = F_(xp, yp),
+                         fipjm = F_(xm, yp);
+#undef F_
+            const int64_t i = x + (int64_t)Lx * y;
+            outx[i] = ((-1.0 / 3.0) * (fip - fim)) - ((1.0 / 12.0) * (((fipjp - fimjp) - fimjm) + fipjm));
+            outy[i] = ((-1.0 / 3.0) * (fjp - fjm)) - ((1.0 / 12.0) * (((fipjp + fimjp) - fimjm) - fipjm));
+        }
+}
+
+/* wetted! (src/measures.jl:6-17), fluid_dry! (:23-28) and surfacearea!
+ * (:126-168) of one height field: out = {count(h > thresh), maximum(h),
+ * Σ sqrt(gx² + gy² + 1) over h > thresh, that sum minus count·cos πθ
+ * (theta_field NULL) or Σ cos πθ over h > thresh}.  Sums run sequentially in
+ * column-major order (the reference's sum over a masked vector).  mask may
+ * be NULL.  Pinned: test/measures.jl:2-97. */
+void orc_measure(const double *h, const double *theta_field, double theta, int Lx, int Ly,
+                 double thresh, double out[4], unsigned char *mask) {
+    const int64_t N = (int64_t)Lx * Ly;
+    double *gx = (double *)malloc(sizeof(double) * N), *gy = (double *)malloc(sizeof(double) * N);
+    orc_grad_simple(gx, gy, h, Lx, Ly);
+    double cnt = 0.0, mx = -INFINITY, surf = 0.0, cs = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        const int wet = h[i] > thresh;
+        mx = h[i] > mx ? h[i] : mx;
+        if (mask) mask[i] = (unsigned char)wet;
+        if (!wet) continue;
+        cnt += 1.0;
+        surf += sqrt((gx[i] * gx[i] + gy[i] * gy[i]) + 1.0);
+        if (theta_field) cs += orc_cospi(theta_field[i]);
+    }
+    out[0] = cnt;
+    out[1] = mx;
+    out[2] = surf;
+    out[3] = surf - (theta_field ? cs : cnt * orc_cospi(theta));
+    free(gx);
+    free(gy);
+}
+
+/* t0(; hᵦ, γ, μ, θ) (src/measures.jl:56-61): the spinodal dewetting time
+ * scale 3μ/(γ q²²) with q² = hᵦ(1 - cos πθ)(2 - 3hᵦ).  Pinned:
+ * test/measures.jl:117-120 (t0() ≈ 177428.32340802). */
+void orc_t0(double hb, double gamma, double mu, double theta, double out[2]) {
+    const double qsq = ((hb * (1.0 - orc_cospi(theta))) * (2.0 - 3.0 * hb));
+    out[0] = (3.0 * mu) / (gamma * (qsq * qsq));
+    out[1] = qsq;
+}

@@ -106,6 +106,11 @@ int orc_time_loop(orc_state *st, const orc_params *p, int64_t nsteps, const orc_
 
 /* ---- one dimension: D1Q3 (State_1D, src/initialize.jl:276-290); fields of L
  * nodes, distributions (L, 3), population k at offset k*L ---- */
+void orc_grad_simple(double *outx, double *outy, const double *f, int Lx, int Ly);
+void orc_measure(const double *h, const double *theta_field, double theta, int Lx, int Ly,
+                 double thresh, double out[4], unsigned char *mask);
+void orc_t0(double hb, double gamma, double mu, double theta, double out[2]);
+
 typedef struct orc1d_state {
     int32_t L;
     double *fout, *ftemp, *feq;

@@ -21,7 +21,7 @@ LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(LIBDIR, "libswalbe_hip.so")
 SOURCES = ["api.hip", "ops.hip", "step.hip", "step_f32.hip", "batch.hip", "serve.hip", "probe.hip",
-           "d1q3.hip"]
+           "d1q3.hip", "measure.hip"]
 ARCH = os.environ.get("SWALBE_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -689,6 +689,36 @@ int swalbe_reduce(swalbe_state* st, int32_t field, double thresh, double out[4])
   return SWALBE_OK;
 }
 
+int swalbe_measure(swalbe_state* st, double thresh, int32_t use_theta_field, double theta,
+                   int32_t surface, double out[4], uint8_t* host_mask, size_t mask_bytes) {
+  SW_CHECK_STATE(st);
+  Film& F = st->film;
+  if (!out || (host_mask && mask_bytes != (size_t)F.N)) {
+    set_error("swalbe_measure: null output or mask size != Lx*Ly");
+    return SWALBE_EINVAL;
+  }
+  if (use_theta_field && !F.theta) {
+    set_error("swalbe_measure: no θ field uploaded");
+    return SWALBE_EINVAL;
+  }
+  if (int rc = set_device(st)) return rc;
+  uint8_t* mask = nullptr;
+  if (host_mask) SW_HIP(hipMalloc((void**)&mask, F.N));
+  SW_BOTH(st, launch_measure<T>((const T*)F.height(), enc_offset(F, SWALBE_HEIGHT, 0), F.Lx, F.Ly,
+                                thresh, surface, cospi(theta),
+                                use_theta_field ? (const T*)F.theta : nullptr, mask, st->partials,
+                                st->nparts, st->red_out, st->stream));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, st->red_out, 4 * sizeof(double), hipMemcpyDeviceToHost, st->stream);
+  if (e == hipSuccess && mask)
+    e = hipMemcpyAsync(host_mask, mask, F.N, hipMemcpyDeviceToHost, st->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+  if (mask) (void)hipFree(mask);
+  if (e != hipSuccess) return hip_fail(e, "swalbe_measure");
+  return SWALBE_OK;
+}
+
 int swalbe_init_height(swalbe_state* st, const swalbe_film_init* init) {
   SW_CHECK_STATE(st);
   if (!init || init->kind < SWALBE_INIT_FLAT || init->kind > SWALBE_INIT_TWO_CAPS) {
@@ -808,10 +838,14 @@ struct LoopBuffers {
   unsigned long long* keys = nullptr;  // 2 per step
   unsigned long long* wet = nullptr;   // 1 per step
   double* mass = nullptr;              // 4 per tdump
+  double* meas = nullptr;              // 4 per step: count, max, area, energy
+  uint8_t* fluid = nullptr;            // Lx*Ly per step
   ~LoopBuffers() {
     if (keys) hipFree(keys);
     if (wet) hipFree(wet);
     if (mass) hipFree(mass);
+    if (meas) hipFree(meas);
+    if (fluid) hipFree(fluid);
   }
 };
 
@@ -897,12 +931,28 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
   }
   const StepParams sp = make_step_params(p, o, 0);
   LoopBuffers lb;
-  const bool want_dh = o->dh_series != nullptr, want_wet = o->wet_series != nullptr;
+  const bool want_maxh = o->maxh_series != nullptr;
+  const bool want_dh = o->dh_series != nullptr || want_maxh, want_wet = o->wet_series != nullptr;
   const bool want_mass = o->mass_series != nullptr && p->tdump > 0;
-  if (ms_out && (want_dh || want_wet || want_mass)) {
+  const bool want_surf = o->area_series != nullptr || o->energy_series != nullptr;
+  const bool want_fluid = o->fluid_series != nullptr;
+  const bool want_meas = want_surf || want_fluid;
+  if (ms_out && (want_dh || want_wet || want_mass || want_meas)) {
     set_error("swalbe_time_loop_timed: series are not recorded by the timed variant");
     return SWALBE_EINVAL;
   }
+  if (want_surf && o->surf_theta_field && !F.theta) {
+    set_error("swalbe_time_loop: surface energy with the θ field, but no θ field uploaded");
+    return SWALBE_EINVAL;
+  }
+  if (want_fluid && (double)nsteps * (double)F.N > 4e9) {
+    set_error("swalbe_time_loop: the fluid_dry series needs nsteps * Lx * Ly bytes on the device "
+              "(more than 4 GB); run fewer steps per call");
+    return SWALBE_EINVAL;
+  }
+  if (want_meas) SW_HIP(hipMalloc((void**)&lb.meas, sizeof(double) * 4 * std::max<int64_t>(1, nsteps)));
+  if (want_fluid) SW_HIP(hipMalloc((void**)&lb.fluid, (size_t)F.N * std::max<int64_t>(1, nsteps)));
+  const double meas_thresh = o->surf_thresh > 0.0 ? o->surf_thresh : 0.055;
   if (want_dh) SW_HIP(hipMalloc(&lb.keys, sizeof(unsigned long long) * 2 * std::max<int64_t>(1, nsteps)));
   if (want_wet) SW_HIP(hipMalloc(&lb.wet, sizeof(unsigned long long) * std::max<int64_t>(1, nsteps)));
   int64_t nmass = 0;
@@ -960,6 +1010,14 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
         return rc;
       ++km;
     }
+    if (want_meas) {  // measurement callbacks on the step's input height, no host sync
+      SW_BOTH(st, launch_measure<T>(
+                      (const T*)F.height(), enc_offset(F, SWALBE_HEIGHT, 0), F.Lx, F.Ly,
+                      meas_thresh, want_surf ? 1 : 0, cospi(p->theta),
+                      want_surf && o->surf_theta_field ? (const T*)F.theta : nullptr,
+                      want_fluid ? lb.fluid + (size_t)F.N * (it - 1) : nullptr, st->partials,
+                      st->nparts, lb.meas + 4 * (it - 1), st->stream));
+    }
     StepSeries ser{want_dh ? lb.keys + 2 * (it - 1) : nullptr, want_wet ? lb.wet + (it - 1) : nullptr};
     if (F.dtype == SWALBE_F64)
       enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal);
@@ -998,8 +1056,22 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
     std::vector<unsigned long long> keys(2 * nsteps);
     SW_HIP(hipMemcpyAsync(keys.data(), lb.keys, keys.size() * 8, hipMemcpyDeviceToHost, st->stream));
     SW_HIP(hipStreamSynchronize(st->stream));
-    for (int64_t i = 0; i < nsteps; ++i)
-      o->dh_series[i] = dkey_inv(keys[2 * i]) - dkey_inv(keys[2 * i + 1]);
+    for (int64_t i = 0; i < nsteps; ++i) {
+      if (o->dh_series) o->dh_series[i] = dkey_inv(keys[2 * i]) - dkey_inv(keys[2 * i + 1]);
+      if (want_maxh) o->maxh_series[i] = dkey_inv(keys[2 * i]);
+    }
+  }
+  if (want_meas && nsteps > 0) {
+    std::vector<double> m(4 * nsteps);
+    SW_HIP(hipMemcpyAsync(m.data(), lb.meas, m.size() * 8, hipMemcpyDeviceToHost, st->stream));
+    if (want_fluid)
+      SW_HIP(hipMemcpyAsync(o->fluid_series, lb.fluid, (size_t)F.N * nsteps,
+                            hipMemcpyDeviceToHost, st->stream));
+    SW_HIP(hipStreamSynchronize(st->stream));
+    for (int64_t i = 0; i < nsteps; ++i) {
+      if (o->area_series) o->area_series[i] = m[4 * i + 2];
+      if (o->energy_series) o->energy_series[i] = m[4 * i + 3];
+    }
   }
   if (want_wet) {
     std::vector<unsigned long long> w(nsteps);

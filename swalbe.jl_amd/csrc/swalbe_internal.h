@@ -160,6 +160,15 @@ void launch_reduce(const T* h, int64_t N, double href, double thresh, double* pa
                    int nparts, hipStream_t s);
 void launch_reduce_finish(const double* partials, int nparts, double* out, hipStream_t s);
 
+// On-device measurements of a height field (href added to every value):
+// out = {count(h > thresh), max h, Σ sqrt(|∇h|² + 1) over h > thresh,
+// reduced energy}; surface = 0 skips the gradient sums; mask (N bytes) gets
+// h > thresh when non-null.  partials holds 4 doubles per block (nparts).
+template <typename T>
+void launch_measure(const T* h, double href, int Lx, int Ly, double thresh, int surface,
+                    double cos_s, const T* theta, uint8_t* mask, double* partials, int nparts,
+                    double* out, hipStream_t s);
+
 // ---------------------------------------------------------------- errors
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);

@@ -61,6 +61,9 @@ class StepOpts(C.Structure):
         ("wet_series", C.POINTER(C.c_int64)), ("mass_series", C.POINTER(C.c_double)),
         ("tile_y", C.c_int32), ("variant", C.c_int32), ("nontemporal", C.c_int32),
         ("pad_", C.c_int32), ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
+        ("maxh_series", C.POINTER(C.c_double)), ("area_series", C.POINTER(C.c_double)),
+        ("energy_series", C.POINTER(C.c_double)), ("fluid_series", C.POINTER(C.c_uint8)),
+        ("surf_thresh", C.c_double), ("surf_theta_field", C.c_int32), ("pad2_", C.c_int32),
     ]
 
 
@@ -143,6 +146,9 @@ def lib():
     _sig(L, "swalbe_reduce", [_V, _I, _D, C.POINTER(_D)])
     _sig(L, "swalbe_init_random", [_V, _I, C.c_uint64, _D, _D])
     _sig(L, "swalbe_init_height", [_V, C.POINTER(FilmInit)])
+    _sig(L, "swalbe_measure", [_V, _D, _I, _D, _I, C.POINTER(_D), _V, C.c_size_t])
+    _sig(L, "swalbe_measure_ptr", [_I, _V, _V, _D, _I, _I, _D, _V, _V, _V])
+    _sig(L, "swalbe_grad_simple_ptr", [_I, _V, _V, _V, _I, _I, _V])
     _sig(L, "swalbe_device_memory", [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)])
     _sig(L, "swalbe_batch_create", [_I, C.c_size_t, _I, C.POINTER(_V)])
     _sig(L, "swalbe_batch_destroy", [_V])

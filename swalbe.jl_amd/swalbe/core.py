@@ -494,14 +494,111 @@ def cospi(x: float) -> float:
 # ---------------------------------------------------------------- time loop
 
 
-def wetted(area_size, state, hthresh: float = 0.055):
-    """wetted!(area_size, state) (src/measures.jl:13-17): push count(h > hthresh)."""
-    area_size.append(int(state.reduce("height", hthresh)["count"]))
+def wetted(area_size, *args, hthresh: float = 0.055):
+    """wetted!(area_size, state) (src/measures.jl:13-17): push count(h > hthresh);
+    wetted!(area_size, maxheight, height, t) (:6-11): area_size[t] and
+    maxheight[t] (t 1-based) of a state or a height array, measured on the GPU."""
+    if len(args) == 1:
+        area_size.append(int(args[0].reduce("height", hthresh)["count"]))
+        return None
+    maxheight, height, t = args
+    m = measure(height, hthresh, surface=False)
+    area_size[t - 1] = m["count"]
+    maxheight[t - 1] = m["max"]
+    return None
+
+
+def measure(height, thresh: float = 0.055, θ=0.0, surface: bool = True, mask=None):
+    """The on-device measurement behind wetted!, fluid_dry! and surfacearea!:
+    {count(h > thresh), max h, liquid-vapour area, reduced energy} of a
+    state's height or of a height array; θ a scalar or a field; `mask` (a
+    uint8 array of the film's shape, Fortran order) receives h > thresh."""
+    if _is1d(height):
+        raise TypeError("the surface measures are defined for 2D films")
+    out = (C.c_double * 4)()
+    if isinstance(height, HipState):
+        use_field = 0
+        th = θ
+        if np.ndim(θ) > 0:
+            height.set("theta", θ)
+            use_field, th = 1, 0.0
+        mb = None if mask is None else mask.ctypes.data
+        check(lib().swalbe_measure(height.handle, float(thresh), use_field, float(th),
+                                   int(bool(surface)), out, mb, 0 if mask is None else mask.nbytes),
+              "measure")
+    else:
+        h = np.asarray(height)
+        s2 = _shape2(h)
+        R = _Raw(_dt([h]), s2)
+        th_f = R.arg(θ, s2) if np.ndim(θ) > 0 else None
+        dmask = DeviceArray(s2, dtype=np.uint8) if mask is not None else None
+        dout = DeviceArray((4,), dtype=np.float64)
+        check(lib().swalbe_measure_ptr(R.dtype, R.arg(h, s2), th_f, float(θ) if th_f is None else 0.0,
+                                       s2[0], s2[1], float(thresh),
+                                       None if dmask is None else dmask.ptr, dout.ptr, None),
+              "measure")
+        R.finish()
+        vals = dout.get()
+        for k in range(4):
+            out[k] = vals[k]
+        if mask is not None:
+            mask[...] = dmask.get().reshape(s2, order="F")
+    return {"count": int(out[0]), "max": out[1], "area": out[2], "energy": out[3]}
+
+
+def fluid_dry(fluid, dummy, height, t: int, hthresh: float = 0.055):
+    """fluid_dry!(fluid, dummy, height, t) (src/measures.jl:23-28): dummy = h > hthresh
+    and fluid[t, :] = vec(dummy) (t 1-based), the mask taken on the GPU."""
+    shape = (height.Lx, height.Ly) if isinstance(height, HipState) else _shape2(np.asarray(height))
+    m = np.zeros(shape, dtype=np.uint8, order="F")
+    measure(height, hthresh, surface=False, mask=m)
+    dummy[...] = m.astype(bool)
+    fluid[t - 1, :] = m.ravel(order="F").astype(bool)
+
+
+def surfacearea(area_lv, red_energy, height, θ, dhx=None, dhy=None, dgrad=None, surface=None,
+                t: int = 1, htresh: float = 0.055):
+    """surfacearea!(area_lv, red_energy, height, θ, ∇hx, ∇hy, dgrad, surface, t)
+    (src/measures.jl:126-168): liquid-vapour area Σ sqrt(|∇h|² + 1) over
+    h > htresh (∇f_simple! gradients) and the reduced energy, stored at t
+    (1-based); θ a scalar or a field.  ∇hx / ∇hy, when given, receive the
+    gradients; dgrad and surface are scratch here."""
+    m = measure(height, htresh, θ=θ, surface=True)
+    area_lv[t - 1] = m["area"]
+    red_energy[t - 1] = m["energy"]
+    if dhx is not None and dhy is not None:
+        h = height.height if isinstance(height, HipState) else height
+        grad_f_simple(dhx, dhy, h, dgrad)
+
+
+def grad_f_simple(outx, outy, f, dgrad=None):
+    """∇f_simple!(outputx, outputy, f, dgrad) (src/measures.jl:175-192)."""
+    s2 = _shape2(f)
+    R = _Raw(_dt([outx, outy, f]), s2)
+    check(lib().swalbe_grad_simple_ptr(R.dtype, R.arg(outx, s2, True), R.arg(outy, s2, True),
+                                       R.arg(f, s2), s2[0], s2[1], None), "grad_f_simple")
+    R.finish()
+
+
+def t0(hᵦ: float = 0.07, γ: float = 0.01, μ: float = 1 / 6, θ: float = 1 / 6, **kw):
+    """t0(; hᵦ, γ, μ, θ) (src/measures.jl:56-61): the spinodal dewetting time
+    scale 3μ/(γ q⁴) with q² = hᵦ(1 - cos πθ)(2 - 3hᵦ); returns (t0, q²)."""
+    hᵦ = kw.pop("hb", hᵦ)
+    γ = kw.pop("gamma", γ)
+    μ = kw.pop("mu", μ)
+    θ = kw.pop("theta", θ)
+    if kw:
+        raise TypeError(f"unknown keywords {sorted(kw)}")
+    qsq = (hᵦ * (1 - cospi(θ))) * (2 - 3 * hᵦ)
+    return (3 * μ) / (γ * (qsq * qsq)), qsq
 
 
 def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, thermal: bool = False,
               seed: int = 0, materialize: bool = True, tile_y: int = 0, mass: Optional[list] = None,
-              snapshots: Optional[dict] = None, snapshot_every: int = 0, **kw1d):
+              snapshots: Optional[dict] = None, snapshot_every: int = 0,
+              maxheight: Optional[list] = None, area_lv: Optional[list] = None,
+              red_energy: Optional[list] = None, fluid: Optional[np.ndarray] = None,
+              measure_thresh: float = 0.055, surface_theta_field: bool = False, **kw1d):
     """time_loop(sys, state), time_loop(sys, state, θ), time_loop(sys, state, Δh)
     and time_loop(sys, state, f, measure) (src/simulate.jl:6-96).
 
@@ -513,7 +610,12 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     (scripts/Rivulet_stability.jl:123-124).  `snapshots` (a dict) receives
     "h_<t>" for every step t divisible by `snapshot_every`, as snapshot!
     (src/measures.jl:99-105) would record them, copied off the device while
-    the following steps run.  A 1D state runs the 1D loop (its own keywords:
+    the following steps run.  maxheight / area_lv / red_energy (lists) and
+    fluid (a (Tmax, Lx*Ly) bool array) receive wetted!'s maximum height,
+    surfacearea!'s area and reduced energy and fluid_dry!'s mask of every
+    step's input height, taken on the device without host synchronisation
+    (threshold measure_thresh; energy from the θ field with
+    surface_theta_field).  A 1D state runs the 1D loop (its own keywords:
     variant, γ, grad_gamma_sign, snapshots + dumping)."""
     if _is1d(state):
         return _one_d().time_loop(sys, state, *args, verbose=verbose, thermal=thermal, seed=seed,
@@ -554,6 +656,17 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     if P.tdump > 0:
         ms = np.zeros(nsteps // P.tdump + 2)
         o.mass_series = ms.ctypes.data_as(C.POINTER(C.c_double))
+    mser = {}
+    for key, lst, field in (("maxh", maxheight, "maxh_series"), ("area", area_lv, "area_series"),
+                            ("energy", red_energy, "energy_series")):
+        if lst is not None:
+            mser[key] = (np.zeros(max(1, nsteps)), lst)
+            setattr(o, field, mser[key][0].ctypes.data_as(C.POINTER(C.c_double)))
+    fl = None
+    if fluid is not None:
+        fl = np.zeros((max(1, nsteps), state.Lx * state.Ly), dtype=np.uint8)
+        o.fluid_series = fl.ctypes.data_as(C.POINTER(C.c_uint8))
+    o.surf_thresh, o.surf_theta_field = float(measure_thresh), int(bool(surface_theta_field))
     snap_t = []
     if snapshots is not None and snapshot_every > 0:
         snap_t = [t for t in range(state.t + 1, state.t + nsteps + 1) if t % snapshot_every == 0]
@@ -565,6 +678,10 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
         snapshots[f"h_{t}"] = snap_buf[k].reshape((state.Lx, state.Ly), order="F").copy()
     t0 = state.t
     state.t = t0 + nsteps
+    for arr, lst in mser.values():
+        lst.extend(arr[:nsteps].tolist())
+    if fl is not None:
+        fluid[:nsteps, :] = fl[:nsteps].astype(bool)
     if ms is not None:
         k = 0
         for t in range(t0 + 1, t0 + nsteps + 1):
