@@ -46,12 +46,18 @@ struct HipStepOpts
     dh_series::Ptr{Float64}; wet_series::Ptr{Int64}; mass_series::Ptr{Float64}
     tile_y::Int32; variant::Int32; nontemporal::Int32; pad_::Int32
     snap_every::Int64; snap_series::Ptr{Float64}
+    maxh_series::Ptr{Float64}; area_series::Ptr{Float64}; energy_series::Ptr{Float64}
+    fluid_series::Ptr{UInt8}; surf_thresh::Float64; surf_theta_field::Int32; pad2_::Int32
 end
 HipStepOpts(; use_theta_field = 0, thermal = 0, body = 0, materialize = 1, seed = 0, t0 = 0,
          body_ax = 0.0, body_ay = 0.0, body_fac = 0.0, wet_thresh = 0.055,
-         dh_series = C_NULL, wet_series = C_NULL, mass_series = C_NULL) =
+         dh_series = C_NULL, wet_series = C_NULL, mass_series = C_NULL,
+         maxh_series = C_NULL, area_series = C_NULL, energy_series = C_NULL,
+         fluid_series = C_NULL, surf_thresh = 0.055, surf_theta_field = 0) =
     HipStepOpts(use_theta_field, thermal, body, materialize, seed, t0, body_ax, body_ay, body_fac,
-             wet_thresh, dh_series, wet_series, mass_series, 0, 0, 0, 0, 0, C_NULL)
+             wet_thresh, dh_series, wet_series, mass_series, 0, 0, 0, 0, 0, C_NULL,
+             maxh_series, area_series, energy_series, fluid_series, surf_thresh,
+             surf_theta_field, 0)
 
 function hip_check(rc::Cint, what)
     rc == HIP_OK && return nothing
@@ -183,7 +189,16 @@ function time_loop(sys::SysConst, s::HipState, Δh::Vector; verbose = false)
     return s
 end
 
-# a measurement callback sees the state between the force sum and
+# wetted! as the callback runs fused: the count of every step is taken on the device
+function time_loop(sys::SysConst, s::HipState, f::typeof(wetted!), measure::Vector;
+                   verbose = false)
+    wet = zeros(Int64, sys.param.Tmax)
+    GC.@preserve wet fused!(sys, s, HipStepOpts(t0 = s.t, wet_series = pointer(wet)))
+    append!(measure, wet)
+    return s, measure
+end
+
+# any other measurement callback sees the state between the force sum and
 # equilibrium!, so it runs the per-op kernels (src/simulate.jl:69-96)
 function time_loop(sys::SysConst, s::HipState, f::Function, measure::Vector; verbose = false)
     for t in 1:sys.param.Tmax
@@ -201,3 +216,202 @@ function time_loop(sys::SysConst, s::HipState, f::Function, measure::Vector; ver
     return s, measure
 end
 
+
+# ---- measurements on the device (src/measures.jl) ----------------------------
+function hip_measure(s::HipState, thresh; θ = 0.0, surface = true, mask = C_NULL, nmask = 0)
+    out = zeros(4)
+    usefield = 0
+    th = θ
+    if !(θ isa Number)
+        s.θ = θ
+        usefield, th = 1, 0.0
+    end
+    hip_check(ccall((:swalbe_measure, libswalbe), Cint,
+                (Ptr{Cvoid}, Float64, Int32, Float64, Int32, Ptr{Float64}, Ptr{UInt8}, Csize_t),
+                s.handle, thresh, usefield, th, surface, out, mask, nmask), "measure")
+    return out  # count, maximum, liquid-vapour area, reduced energy
+end
+
+# wetted!(area_size, maxheight, height, t; hthresh) (src/measures.jl:6-11)
+function wetted!(area_size, maxheight, s::HipState, t; hthresh = 0.055)
+    m = hip_measure(s, hthresh; surface = 0)
+    area_size[t] = m[1]
+    maxheight[t] = m[2]
+    return nothing
+end
+# wetted!(area_size, state; hthresh) (:13-17)
+function wetted!(area_size, s::HipState; hthresh = 0.055)
+    push!(area_size, Int(hip_measure(s, hthresh; surface = 0)[1]))
+    return nothing
+end
+# fluid_dry!(fluid, dummy, height, t; hthresh) (:23-28)
+function fluid_dry!(fluid, dummy, s::HipState, t; hthresh = 0.055)
+    m = zeros(UInt8, s.Lx * s.Ly)
+    GC.@preserve m hip_measure(s, hthresh; surface = 0, mask = pointer(m), nmask = length(m))
+    dummy .= reshape(m .== 1, s.Lx, s.Ly)
+    fluid[t, :] .= vec(dummy)
+    return nothing
+end
+# surfacearea!(area_lv, red_energy, height, θ, ∇hx, ∇hy, dgrad, surface, t; htresh) (:126-168)
+function surfacearea!(area_lv, red_energy, s::HipState, θ, ∇hx, ∇hy, dgrad, surface, t;
+                      htresh = 0.055)
+    m = hip_measure(s, htresh; θ = θ)
+    area_lv[t] = m[3]
+    red_energy[t] = m[4]
+    return nothing
+end
+
+# ---- one-dimensional films (D1Q3) ---------------------------------------------
+# Sys(sysc::Consts_1D; T, kind) gains a `device = "HIP"` keyword branch:
+#     device == "HIP" && return HipState1D{T}(sysc.L; kind = kind)
+const HIP1D_FIELDS = Dict(:fout => 0, :ftemp => 1, :feq => 2, :height => 3, :vel => 4,
+                      :pressure => 5, :F => 6, :slip => 7, :h∇p => 8, :kbt => 9, :θ => 10,
+                      :γ => 11, :∇γ => 12, :border1 => 13, :border2 => 14, :interior => 15)
+
+struct HipStepOpts1D
+    use_theta_field::Int32; use_gamma_field::Int32; grad_gamma::Int32; thermal::Int32
+    body::Int32; bound::Int32; materialize::Int32; variant::Int32
+    seed::UInt64; t0::Int64
+    alpha::Float64; body_fac::Float64; wet_thresh::Float64
+    dh_series::Ptr{Float64}; wet_series::Ptr{Int64}; mass_series::Ptr{Float64}
+    snap_every::Int64; snap_series::Ptr{Float64}
+    threads::Int32; pad_::Int32
+end
+HipStepOpts1D(; use_theta_field = 0, use_gamma_field = 0, grad_gamma = 0, thermal = 0, body = 0,
+           bound = 0, materialize = 1, variant = 0, seed = 0, t0 = 0, alpha = 0.0,
+           body_fac = 0.0, wet_thresh = 0.055, dh_series = C_NULL, wet_series = C_NULL,
+           mass_series = C_NULL, snap_every = 0, snap_series = C_NULL) =
+    HipStepOpts1D(use_theta_field, use_gamma_field, grad_gamma, thermal, body, bound, materialize,
+               variant, seed, t0, alpha, body_fac, wet_thresh, dh_series, wet_series,
+               mass_series, snap_every, snap_series, 0, 0)
+
+"""GPU-resident State_1D (and its thermal / gamma kinds): the library owns the
+arrays; `s.basestate` is `s` itself."""
+mutable struct HipState1D{T} <: LBM_state_1D
+    handle::Ptr{Cvoid}
+    L::Int
+    t::Int
+end
+function HipState1D{T}(L; kind = "simple") where {T}
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    hip_check(ccall((:swalbe1d_create, libswalbe), Cint, (Int32, Int32, Int32, Ref{Ptr{Cvoid}}),
+                L, T === Float32 ? HIP_F32 : HIP_F64,
+                kind == "thermal" ? HIP_THERMAL : HIP_SIMPLE, h), "Sys (1D)")
+    x = HipState1D{T}(h[], L, 0)
+    finalizer(x) do x
+        x.handle == C_NULL || ccall((:swalbe1d_destroy, libswalbe), Cint, (Ptr{Cvoid},), x.handle)
+    end
+    return x
+end
+function Base.getproperty(s::HipState1D{T}, name::Symbol) where {T}
+    name in (:handle, :L, :t) && return getfield(s, name)
+    name === :basestate && return s
+    A = name in (:fout, :ftemp, :feq) ? zeros(Float64, s.L, 3) : zeros(Float64, s.L)
+    hip_check(ccall((:swalbe1d_download, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
+                s.handle, HIP1D_FIELDS[name], A, sizeof(A)), "download $name")
+    return T.(A)
+end
+function Base.setproperty!(s::HipState1D, name::Symbol, A)
+    name in (:handle, :L, :t) && return setfield!(s, name, A)
+    if A isa Number
+        return hip_check(ccall((:swalbe1d_fill, libswalbe), Cint, (Ptr{Cvoid}, Int32, Float64),
+                           s.handle, HIP1D_FIELDS[name], A), "fill $name")
+    end
+    B = Float64.(A)
+    hip_check(ccall((:swalbe1d_upload, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
+                s.handle, HIP1D_FIELDS[name], B, sizeof(B)), "upload $name")
+end
+
+function hip_walls!(s::HipState1D, sys::SysConstWithBound_1D)
+    s.border1 = sys.border[1]
+    s.border2 = sys.border[2]
+    s.interior = sys.interior
+end
+
+equilibrium!(s::HipState1D, sys::Consts_1D; g = sys.param.g) =
+    hip_check(ccall((:swalbe1d_equilibrium, libswalbe), Cint, (Ptr{Cvoid}, Float64), s.handle, g),
+          "equilibrium!")
+BGKandStream!(s::HipState1D, sys::SysConst_1D) =
+    hip_check(ccall((:swalbe1d_bgk_and_stream, libswalbe), Cint, (Ptr{Cvoid}, Float64, Int32),
+                s.handle, sys.param.τ, 0), "BGKandStream!")
+function BGKandStream!(s::HipState1D, sys::SysConstWithBound_1D)
+    hip_walls!(s, sys)
+    hip_check(ccall((:swalbe1d_bgk_and_stream, libswalbe), Cint, (Ptr{Cvoid}, Float64, Int32),
+                s.handle, sys.param.τ, 1), "BGKandStream!")
+end
+moments!(s::HipState1D) =
+    hip_check(ccall((:swalbe1d_moments, libswalbe), Cint, (Ptr{Cvoid},), s.handle), "moments!")
+function filmpressure!(s::HipState1D, sys::Consts_1D; θ = sys.param.θ, n = sys.param.n,
+                       m = sys.param.m, hmin = sys.param.hmin, hcrit = sys.param.hcrit,
+                       γ = sys.param.γ)
+    usθ, usγ = 0, 0
+    if !(θ isa Number); s.θ = θ; usθ, θ = 1, 0.0; end
+    if !(γ isa Number); s.γ = γ; usγ, γ = 1, 0.0; end
+    hip_check(ccall((:swalbe1d_filmpressure, libswalbe), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Int32, Int32, Float64, Float64, Int32, Int32, Int32),
+                s.handle, γ, θ, n, m, hmin, hcrit, usθ, usγ, 0), "filmpressure!")
+end
+h∇p!(s::HipState1D) =
+    hip_check(ccall((:swalbe1d_hgradp, libswalbe), Cint, (Ptr{Cvoid},), s.handle), "h∇p!")
+slippage!(s::HipState1D, sys::Consts_1D) =
+    hip_check(ccall((:swalbe1d_slippage, libswalbe), Cint, (Ptr{Cvoid}, Float64, Float64),
+                s.handle, sys.param.δ, sys.param.μ), "slippage!")
+thermal!(s::HipState1D, sys::Consts_1D; seed = 0) =
+    hip_check(ccall((:swalbe1d_thermal, libswalbe), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, UInt64, UInt64), s.handle,
+                sys.param.kbt, sys.param.μ, sys.param.δ, seed, s.t), "thermal!")
+inclination!(α::Float64, s::HipState1D; t = 1000, tstart = 0, tsmooth = 1) =
+    hip_check(ccall((:swalbe1d_inclination, libswalbe), Cint, (Ptr{Cvoid}, Float64, Float64),
+                s.handle, α, 0.5 + 0.5 * tanh((t - tstart) / tsmooth)), "inclination!")
+∇γ!(s::HipState1D) =
+    hip_check(ccall((:swalbe1d_grad_gamma, libswalbe), Cint, (Ptr{Cvoid}, Int32, Float64),
+                s.handle, 0, 1.0), "∇γ!")
+∇γ!(s::HipState1D, sys::SysConst_1D) =
+    hip_check(ccall((:swalbe1d_grad_gamma, libswalbe), Cint, (Ptr{Cvoid}, Int32, Float64),
+                s.handle, 1, sys.param.δ), "∇γ!")
+
+function fused1d!(sys, s::HipState1D, o::HipStepOpts1D; theta = sys.param.θ, gamma = sys.param.γ)
+    p = HipParams(sys.param)
+    p = HipParams(p.Tmax, p.tdump, p.tau, p.cs, p.mu, p.delta, p.kbt, gamma, p.n, p.m, p.hmin,
+                p.hcrit, theta, p.g)
+    hip_check(ccall((:swalbe1d_time_loop, libswalbe), Cint,
+                (Ptr{Cvoid}, Ref{HipParams}, Int64, Ref{HipStepOpts1D}), s.handle, p,
+                sys.param.Tmax, o), "time_loop")
+    s.t += sys.param.Tmax
+    return s
+end
+
+# time_loop(sys::SysConst_1D, state[, θ | Δh | f, measure]) (src/simulate.jl:98-178)
+time_loop(sys::SysConst_1D, s::HipState1D; verbose = false) =
+    fused1d!(sys, s, HipStepOpts1D(t0 = s.t))
+function time_loop(sys::SysConst_1D, s::HipState1D, θ; verbose = false)
+    θ isa Number && return fused1d!(sys, s, HipStepOpts1D(t0 = s.t); theta = θ)
+    s.θ = θ
+    return fused1d!(sys, s, HipStepOpts1D(use_theta_field = 1, t0 = s.t))
+end
+function time_loop(sys::SysConst_1D, s::HipState1D, Δh::Vector; verbose = false)
+    dh = zeros(sys.param.Tmax)
+    GC.@preserve dh fused1d!(sys, s, HipStepOpts1D(t0 = s.t, dh_series = pointer(dh)))
+    append!(Δh, dh)
+    return s
+end
+function time_loop(sys::SysConst_1D, s::HipState1D, f::typeof(inclination!), α; verbose = false)
+    fused1d!(sys, s, HipStepOpts1D(t0 = s.t, body = 1, alpha = α, body_fac = 1.0))
+    return s, α
+end
+function time_loop(sys::SysConst_1D, s::HipState1D, f::Function, measure; verbose = false)
+    for t in 1:sys.param.Tmax
+        filmpressure!(s, sys); h∇p!(s); slippage!(s, sys)
+        hip_check(ccall((:swalbe1d_force_sum, libswalbe), Cint, (Ptr{Cvoid}, Int32, Int32),
+                    s.handle, 0, 0), "force sum")
+        f(measure, s)
+        equilibrium!(s, sys); BGKandStream!(s, sys); moments!(s)
+        s.t += 1
+    end
+    return s, measure
+end
+# the walled loop (src/simulate.jl:181-205)
+function time_loop(sys::SysConstWithBound_1D, s::HipState1D; verbose = false, diff = 0.05)
+    hip_walls!(s, sys)
+    return fused1d!(sys, s, HipStepOpts1D(t0 = s.t, bound = 1))
+end

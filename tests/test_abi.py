@@ -74,3 +74,31 @@ def test_singledroplet_reference_kat():
         singledroplet = staticmethod(S.singledroplet)
         cospi = staticmethod(S.cospi)
     kat.run_case(B, case)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # sizeof / offsetof of every field of the option structs, from gcc on the
+    # header, against the ctypes mirrors (and so the field order the Julia
+    # binding copies)
+    import ctypes as C
+    import subprocess
+
+    from swalbe import _lib
+    structs = {"swalbe_params": _lib.Params, "swalbe_step_opts": _lib.StepOpts,
+               "swalbe1d_step_opts": _lib.StepOpts1D, "swalbe_film_init": _lib.FilmInit}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "swalbe_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (ln.split() for ln in out if ln)}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
