@@ -34,7 +34,7 @@ WORKLOADS = {
 }
 # many-film workloads (FP32 batched kernel), the 1D coalescence scan and the copy calibration
 BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson", "copy4096_f32", "pattern4096_f32",
-                   "coalesce1d_scan", "single1d_f64")
+                   "coalesce1d_scan", "single1d_f64", "long1d_f64")
 
 
 def bytes_per_lup(esz: int, tau_is_one: bool) -> int:
@@ -243,6 +243,46 @@ def bench_1d(a):
     print(json.dumps(line), flush=True)
 
 
+def bench_long1d(a):
+    """One long 1D film (L = 2^24, FP64, periodic, h = 1 + 0.01 N(0,1)), one
+    launch per step (k1_step): 64 algorithmic bytes per node update (reads
+    h, u; writes f0..f2, h, u, p)."""
+    import ctypes as C
+    import numpy as np
+    import swalbe as S
+    from swalbe import _lib
+    L = 1 << 24
+    sysc = S.SysConst_1D(L=L, param=S.Taumucs(Tmax=a.steps, tdump=10 ** 9))
+    st = S.Sys(sysc, T=np.float64)
+    z0, _ = S.normals(L, 1234)
+    st.height = 1.0 + 0.01 * z0
+    S.equilibrium(st, sysc)
+    prm = sysc.param.cparams()
+    o = _lib.StepOpts1D()
+    o.variant, o.materialize = 1, 0
+    ms = C.c_double()
+    o.t0 = 0
+    _lib.check(_lib.lib().swalbe1d_time_loop_timed(st.handle, C.byref(prm), a.warmup, C.byref(o),
+                                                   C.byref(ms)), "1d loop")
+    o.t0 = a.warmup
+    _lib.check(_lib.lib().swalbe1d_time_loop_timed(st.handle, C.byref(prm), a.steps, C.byref(o),
+                                                   C.byref(ms)), "1d loop")
+    t = ms.value / 1e3
+    glups = L * a.steps / t / 1e9
+    achieved = 64 * L * a.steps / t / 1e9
+    line = {"metric": "GLUPS of the 1D D1Q3 step on one long film (one launch per step)",
+            "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(ms.value / a.steps, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded Philox film h0 = 1 + 0.01 N(0,1))",
+            "config": {"workload": f"long1d_f64: L = {L}, FP64, periodic, tau = 1, k1_step",
+                       "L": L, "finite": bool(np.isfinite(st.height[:4096]).all())},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": None, "bytes_per_lup": 64}}
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(a):
     """Device-to-device copy of the nine distribution planes of a 4096^2 FP32
     state (feq -> ftemp, 604 MB each way): the achievable HBM bandwidth of a
@@ -402,6 +442,8 @@ def main():
         return bench_pattern(a)
     if a.workload in ("coalesce1d_scan", "single1d_f64"):
         return bench_1d(a)
+    if a.workload == "long1d_f64":
+        return bench_long1d(a)
 
     import ctypes as C
     import numpy as np
