@@ -179,7 +179,10 @@ typedef struct swalbe_step_opts {
     uint8_t *fluid_series;  /* host [nsteps * Lx * Ly]: h > surf_thresh per step */
     double surf_thresh;     /* htresh / hthresh of those (0 = 0.055) */
     int32_t surf_theta_field; /* reduced energy from the θ field (else cos πθ of params) */
-    int32_t pad2_;
+    /* 1: with τ = 1, write the distributions and the pressure on the last
+     * step only (no step reads them, so every result is the same); the
+     * other fields are written every step.  Ignored when τ ≠ 1. */
+    int32_t final_outputs_only;
 } swalbe_step_opts;
 int swalbe_time_loop(swalbe_state *st, const swalbe_params *p, int64_t nsteps,
                      const swalbe_step_opts *opts);

@@ -851,7 +851,7 @@ struct LoopBuffers {
 
 template <typename T>
 void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64_t t, int tile_y,
-                  int variant, int nts) {
+                  int variant, int nts, bool outs) {
   Film& F = st->film;
   const int cur = F.hu_cur;
   const int fdst = F.fout_buf != F.ftemp_buf ? F.fout_buf : 1 - F.ftemp_buf;
@@ -866,6 +866,10 @@ void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64
   io.uy_out = (T*)F.uybuf[1 - cur];
   io.p_out = (T*)F.pressure;
   io.kappa = sp.use_theta ? (const T*)F.kappa : nullptr;
+  if (!outs) {  // distributions and pressure of a step nobody reads (τ = 1)
+    io.f_out = nullptr;
+    io.p_out = nullptr;
+  }
   if constexpr (sizeof(T) == 8)
     launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.fstride, F.enc, tile_y, st->stream);
   else
@@ -930,6 +934,9 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
     return SWALBE_EINVAL;
   }
   const StepParams sp = make_step_params(p, o, 0);
+  // with τ = 1 a step reads no distributions, so those of every step but the
+  // last (and its pressure) are dead unless the caller wants them written
+  const bool final_only = o->final_outputs_only && !sp.read_ft;
   LoopBuffers lb;
   const bool want_maxh = o->maxh_series != nullptr;
   const bool want_dh = o->dh_series != nullptr || want_maxh, want_wet = o->wet_series != nullptr;
@@ -1019,10 +1026,11 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
                       st->nparts, lb.meas + 4 * (it - 1), st->stream));
     }
     StepSeries ser{want_dh ? lb.keys + 2 * (it - 1) : nullptr, want_wet ? lb.wet + (it - 1) : nullptr};
+    const bool outs = !final_only || it == nsteps;
     if (F.dtype == SWALBE_F64)
-      enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal);
+      enqueue_step<double>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal, outs);
     else
-      enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal);
+      enqueue_step<float>(st, sp, ser, (uint64_t)t, tile_y, o->variant, o->nontemporal, outs);
     if (want_snap && t % o->snap_every == 0) {
       const int r = (int)(ksnap % SnapRing::R);
       if (int rc = snap_drain(r)) return rc;

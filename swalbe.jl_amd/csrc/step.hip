@@ -272,7 +272,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
       }
       if (k > 0) f = f + force_term<T>(k, sh_fx[su], sh_fy[su]);
       fo[k] = f;
-      io.f_out[k * FS + xo] = f;
+      if (io.f_out) io.f_out[k * FS + xo] = f;
     }
     T s = fo[0];
 #pragma unroll
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
     io.h_out[xo] = s;
     io.ux_out[xo] = (((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7]) + fo[8]) / hnew;
     io.uy_out[xo] = (((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7]) - fo[8]) / hnew;
-    io.p_out[xo] = sh_p[(row + 2) * PC + (tx + 2)];
+    if (io.p_out) io.p_out[xo] = sh_p[(row + 2) * PC + (tx + 2)];
     if (ser.dh_keys || ser.wet) {
       const T hc = sh_h[(row + 3) * HC + (tx + 3)];
       const double hv = DEV ? (double)href + (double)hc : (double)hc;

@@ -310,14 +310,14 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
         }
       }
       fo[k] = f;
-      store_f(&io.f_out[k * FS + xo], f, R.nts);
+      if (io.f_out) store_f(&io.f_out[k * FS + xo], f, R.nts);
     }
     const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) + fo[8];
     const float ih = __builtin_amdgcn_rcpf(K.href + s);
     store_f(&io.h_out[xo], s, R.nts);
     store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, R.nts);
     store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, R.nts);
-    store_f(&io.p_out[xo], sm.p[(row + 2) * PC + (lane + 2)], R.nts);
+    if (io.p_out) store_f(&io.p_out[xo], sm.p[(row + 2) * PC + (lane + 2)], R.nts);
     if constexpr (SERIES) {
       const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (lane + 3)];
       acc.lmax = hv > acc.lmax ? hv : acc.lmax;
@@ -497,15 +497,17 @@ __device__ __forceinline__ void strip_run(const TileIO& io, const F32K& K, const
       fo[8] = at_xm1(fN[8]);
       if (out_lane) {
         const int64_t xo = (int64_t)Lx * j + xc;
+        if (io.f_out) {
 #pragma unroll
-        for (int k = 0; k < kQ; ++k) store_f(&io.f_out[k * FS + xo], fo[k], R.nts);
+          for (int k = 0; k < kQ; ++k) store_f(&io.f_out[k * FS + xo], fo[k], R.nts);
+        }
         const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) +
                         fo[8];
         const float ih = __builtin_amdgcn_rcpf(K.href + s);
         store_f(&io.h_out[xo], s, R.nts);
         store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, R.nts);
         store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, R.nts);
-        store_f(&io.p_out[xo], p0.v, R.nts);
+        if (io.p_out) store_f(&io.p_out[xo], p0.v, R.nts);
       }
     }
   };

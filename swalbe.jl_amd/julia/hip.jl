@@ -47,17 +47,19 @@ struct HipStepOpts
     tile_y::Int32; variant::Int32; nontemporal::Int32; pad_::Int32
     snap_every::Int64; snap_series::Ptr{Float64}
     maxh_series::Ptr{Float64}; area_series::Ptr{Float64}; energy_series::Ptr{Float64}
-    fluid_series::Ptr{UInt8}; surf_thresh::Float64; surf_theta_field::Int32; pad2_::Int32
+    fluid_series::Ptr{UInt8}; surf_thresh::Float64; surf_theta_field::Int32
+    final_outputs_only::Int32
 end
 HipStepOpts(; use_theta_field = 0, thermal = 0, body = 0, materialize = 1, seed = 0, t0 = 0,
          body_ax = 0.0, body_ay = 0.0, body_fac = 0.0, wet_thresh = 0.055,
          dh_series = C_NULL, wet_series = C_NULL, mass_series = C_NULL,
          maxh_series = C_NULL, area_series = C_NULL, energy_series = C_NULL,
-         fluid_series = C_NULL, surf_thresh = 0.055, surf_theta_field = 0) =
+         fluid_series = C_NULL, surf_thresh = 0.055, surf_theta_field = 0,
+         final_outputs_only = 0) =
     HipStepOpts(use_theta_field, thermal, body, materialize, seed, t0, body_ax, body_ay, body_fac,
              wet_thresh, dh_series, wet_series, mass_series, 0, 0, 0, 0, 0, C_NULL,
              maxh_series, area_series, energy_series, fluid_series, surf_thresh,
-             surf_theta_field, 0)
+             surf_theta_field, final_outputs_only)
 
 function hip_check(rc::Cint, what)
     rc == HIP_OK && return nothing

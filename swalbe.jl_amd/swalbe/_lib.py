@@ -63,7 +63,7 @@ class StepOpts(C.Structure):
         ("pad_", C.c_int32), ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
         ("maxh_series", C.POINTER(C.c_double)), ("area_series", C.POINTER(C.c_double)),
         ("energy_series", C.POINTER(C.c_double)), ("fluid_series", C.POINTER(C.c_uint8)),
-        ("surf_thresh", C.c_double), ("surf_theta_field", C.c_int32), ("pad2_", C.c_int32),
+        ("surf_thresh", C.c_double), ("surf_theta_field", C.c_int32), ("final_outputs_only", C.c_int32),
     ]
 
 

@@ -598,7 +598,8 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
               snapshots: Optional[dict] = None, snapshot_every: int = 0,
               maxheight: Optional[list] = None, area_lv: Optional[list] = None,
               red_energy: Optional[list] = None, fluid: Optional[np.ndarray] = None,
-              measure_thresh: float = 0.055, surface_theta_field: bool = False, **kw1d):
+              measure_thresh: float = 0.055, surface_theta_field: bool = False,
+              final_outputs_only: bool = False, variant: int = 0, **kw1d):
     """time_loop(sys, state), time_loop(sys, state, θ), time_loop(sys, state, Δh)
     and time_loop(sys, state, f, measure) (src/simulate.jl:6-96).
 
@@ -615,11 +616,15 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     surfacearea!'s area and reduced energy and fluid_dry!'s mask of every
     step's input height, taken on the device without host synchronisation
     (threshold measure_thresh; energy from the θ field with
-    surface_theta_field).  A 1D state runs the 1D loop (its own keywords:
+    surface_theta_field).  final_outputs_only (τ = 1) writes the
+    distributions and the pressure on the last step only: the same results
+    with 28 instead of 64 bytes per node and step in FP32.  A 1D state runs
+    the 1D loop (its own keywords:
     variant, γ, grad_gamma_sign, snapshots + dumping)."""
     if _is1d(state):
         return _one_d().time_loop(sys, state, *args, verbose=verbose, thermal=thermal, seed=seed,
-                                  materialize=materialize, mass=mass, snapshots=snapshots, **kw1d)
+                                  materialize=materialize, mass=mass, snapshots=snapshots,
+                                  variant=variant, **kw1d)
     if kw1d:
         raise TypeError(f"unknown keywords {sorted(kw1d)}")
     P = sys.param
@@ -627,6 +632,7 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     nsteps = P.Tmax
     o = _lib.StepOpts()
     o.thermal, o.seed, o.t0, o.materialize, o.tile_y = int(thermal), int(seed), int(state.t), int(materialize), tile_y
+    o.variant = int(variant)  # FP32 kernel: 0 default, 1 / 2 LDS tiles, 3 strips
     keep = []
     dh = None
     f = measure = None
@@ -667,6 +673,7 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
         fl = np.zeros((max(1, nsteps), state.Lx * state.Ly), dtype=np.uint8)
         o.fluid_series = fl.ctypes.data_as(C.POINTER(C.c_uint8))
     o.surf_thresh, o.surf_theta_field = float(measure_thresh), int(bool(surface_theta_field))
+    o.final_outputs_only = int(bool(final_outputs_only))
     snap_t = []
     if snapshots is not None and snapshot_every > 0:
         snap_t = [t for t in range(state.t + 1, state.t + nsteps + 1) if t % snapshot_every == 0]

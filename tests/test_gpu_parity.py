@@ -386,3 +386,32 @@ def test_fp32_series_match_oracle(S, variant):
     assert np.all(np.abs(dh - ref_dh) <= 1e-7 + 1e-4 * ref_dh), np.max(np.abs(dh - ref_dh))
     # counts may differ only where h sits within FP32 rounding of the threshold
     assert np.max(np.abs(wet - ref_wet)) <= 3, np.max(np.abs(wet - ref_wet))
+
+
+@pytest.mark.parametrize("T,variant", [(np.float64, 0), (np.float32, 1), (np.float32, 2),
+                                       (np.float32, 3)], ids=["f64", "f32v1", "f32v2", "f32v3"])
+@pytest.mark.parametrize("tau", [1.0, 0.8], ids=["tau1", "tau0.8"])
+def test_final_outputs_only_gives_the_same_state(S, T, variant, tau):
+    """final_outputs_only writes f and p on the last step only (τ = 1; ignored
+    otherwise): every field after the loop, the series and the snapshots are
+    the same bits as with every step written."""
+    Lx, Ly, n = 130, 66, 40
+    h0 = O.init_normal(Lx, Ly, 4, 1.0, 0.01)
+    got = []
+    for final in (False, True):
+        sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=n, tdump=10, kbt=1e-6, τ=tau))
+        st = S.Sys(sysc, "GPU", T=T)
+        st.height = h0
+        S.equilibrium(st, sysc)
+        st.copy("ftemp", "feq")
+        dh, mass, snaps = [], [], {}
+        S.time_loop(sysc, st, dh, thermal=True, seed=3, tile_y=0, mass=mass, snapshots=snaps,
+                    snapshot_every=7, final_outputs_only=final, variant=variant)
+        fields_ = {k: st.get(k) for k in ("height", "velx", "vely", "pressure", "fout", "ftemp",
+                                          "feq", "Fx", "Fy")}
+        got.append((fields_, dh, mass, snaps))
+    (a, dha, ma, sa), (b, dhb, mb, sb) = got
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert dha == dhb and ma == mb
+    assert sa.keys() == sb.keys() and all(np.array_equal(sa[k], sb[k]) for k in sa)
