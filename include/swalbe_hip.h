@@ -255,6 +255,12 @@ int swalbe_batch_step_timed(swalbe_batch *b, int64_t nsteps, double *ms);
 /* replay runs of steps_per_graph steps as one captured HIP graph (0 = off) */
 int swalbe_batch_set_graph(swalbe_batch *b, int32_t steps_per_graph);
 int swalbe_batch_set_nontemporal(swalbe_batch *b, int32_t on);
+/* Films admitted after this call with τ = 1 write their distributions and
+ * pressure on their last step only (their heights and velocities every
+ * step): the same final state for less traffic.  Downloading FOUT or
+ * PRESSURE of such a film before it finishes is an error.  The native
+ * server sets it (its requests return heights). */
+int swalbe_batch_set_final_outputs_only(swalbe_batch *b, int32_t on);
 /* block until at most `steps_behind` of the enqueued batch steps are still
  * pending on the device (bounds how far the host runs ahead) */
 int swalbe_batch_wait(swalbe_batch *b, int64_t steps_behind);

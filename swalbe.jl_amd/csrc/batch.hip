@@ -112,6 +112,7 @@ struct swalbe_batch {
   int nactive = 0;
   int grid = 0;
   int nts = 0;
+  int last_only = 0;  // films admitted with distributions / pressure written at their end only
   int graph_steps = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -373,6 +374,12 @@ int swalbe_batch_set_nontemporal(swalbe_batch* b, int32_t on) {
   return SWALBE_OK;
 }
 
+int swalbe_batch_set_final_outputs_only(swalbe_batch* b, int32_t on) {
+  if (!b) return bad("null batch");
+  b->last_only = on ? 1 : 0;
+  return SWALBE_OK;
+}
+
 int swalbe_batch_add(swalbe_batch* b, const swalbe_params* p, int32_t Lx, int32_t Ly,
                      const swalbe_film_init* init, int64_t nsteps, int32_t* slot_out) {
   if (!b || !p || !init || !slot_out || Lx < 1 || Ly < 1 || nsteps < 0)
@@ -432,6 +439,7 @@ int swalbe_batch_add(swalbe_batch* b, const swalbe_params* p, int32_t Lx, int32_
     d.pairs = (!d.small && (Ly % 2) == 0) ? 1 : 0;
   }
   d.read_ft = two_f ? 1 : 0;
+  d.last_only = b->last_only;
   d.thermal = init->thermal ? 1 : 0;
   d.body = init->body ? 1 : 0;
   d.seed = init->seed;
@@ -545,6 +553,10 @@ int swalbe_batch_download(swalbe_batch* b, int32_t slot, int32_t field, void* ho
     case SWALBE_FOUT: src = d.read_ft ? d.f[cur] : d.f[0]; count = 9 * N; break;
     default: return bad("swalbe_batch_download: field must be HEIGHT, VELX, VELY, PRESSURE or FOUT");
   }
+  if (d.last_only && !d.read_ft && (field == SWALBE_FOUT || field == SWALBE_PRESSURE) &&
+      b->counter < d.end)
+    return bad("swalbe_batch_download: this film writes its distributions and pressure on its "
+               "last step only (swalbe_batch_set_final_outputs_only); they are not ready yet");
   size_t hes;
   if (bytes == (size_t)count * 8) hes = 8;
   else if (bytes == (size_t)count * 4) hes = 4;

@@ -104,6 +104,9 @@ int swalbe_serve_create(size_t capacity_bytes, int32_t max_films, int32_t tick_s
     delete s;
     return rc;
   }
+  // requests return heights only: distributions and pressure of a film are
+  // written on its last step only
+  swalbe_batch_set_final_outputs_only(s->batch, 1);
   if (graph_steps > 0) swalbe_batch_set_graph(s->batch, graph_steps);
   s->max_films = max_films;
   s->tick = tick_steps;

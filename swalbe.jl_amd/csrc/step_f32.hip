@@ -704,6 +704,10 @@ __global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ 
       io.ft = nullptr;
       io.f_out = F.f[0];
     }
+    if (F.last_only && !F.read_ft && L + 1 != F.end) {  // outputs nobody reads before the end
+      io.f_out = nullptr;
+      io.p_out = nullptr;
+    }
     TileRun R;
     R.Lx = F.Lx;
     R.Ly = F.Ly;
@@ -769,6 +773,10 @@ __global__ __launch_bounds__(64) void k_batch_strip_f32(const BatchFilm* __restr
     } else {
       io.ft = nullptr;
       io.f_out = F.f[0];
+    }
+    if (F.last_only && !F.read_ft && L + 1 != F.end) {  // outputs nobody reads before the end
+      io.f_out = nullptr;
+      io.p_out = nullptr;
     }
     TileRun R;
     R.Lx = F.Lx;

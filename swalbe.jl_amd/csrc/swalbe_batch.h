@@ -37,7 +37,8 @@ struct BatchFilm {
   float* p;
   float* f[2];  // f[1] only when τ ≠ 1
   // ntx, ntiles: strips and (strip, segment) items, or tile columns and tiles
-  int Lx, Ly, ntx, ntiles, small, pairs, read_ft, thermal, body, pad_;
+  int Lx, Ly, ntx, ntiles, small, pairs, read_ft, thermal, body;
+  int last_only;  // τ = 1: distributions and pressure written on the last step only
   int64_t fstride;
   uint64_t seed;
   int64_t t0;                // step number before admission

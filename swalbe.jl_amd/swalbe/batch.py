@@ -53,14 +53,23 @@ def film_bytes(Lx: int, Ly: int, tau: float = 1.0) -> int:
 class Batch:
     """A device arena of `capacity_bytes` holding up to `max_films` films."""
 
-    def __init__(self, capacity_bytes: int, max_films: int = 4096, graph_steps: int = 0):
+    def __init__(self, capacity_bytes: int, max_films: int = 4096, graph_steps: int = 0,
+                 final_outputs_only: bool = False):
         h = C.c_void_p()
         check(lib().swalbe_batch_create(_lib.F32, int(capacity_bytes), int(max_films), C.byref(h)),
               "swalbe_batch_create")
         self._h = h
         self.dims = {}
+        if final_outputs_only:
+            self.set_final_outputs_only(True)
         if graph_steps:
             self.set_graph(graph_steps)
+
+    def set_final_outputs_only(self, on: bool = True) -> None:
+        """Films added afterwards (τ = 1) write distributions and pressure on
+        their last step only; heights and velocities every step."""
+        check(lib().swalbe_batch_set_final_outputs_only(self._h, int(bool(on))),
+              "set_final_outputs_only")
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -132,9 +132,12 @@ def poisson_requests(n: int, rate: float = 200.0, seed: int = 5,
 class BatchBackend:
     """The device side of the server: a swalbe Batch."""
 
-    def __init__(self, capacity_bytes: int, max_films: int = 4096, graph_steps: int = 0):
+    def __init__(self, capacity_bytes: int, max_films: int = 4096, graph_steps: int = 0,
+                 final_outputs_only: bool = True):
         from .batch import Batch, film_bytes
-        self.batch = Batch(capacity_bytes, max_films, graph_steps)
+        # requests return heights: a film's distributions and pressure are
+        # written on its last step only
+        self.batch = Batch(capacity_bytes, max_films, graph_steps, final_outputs_only)
         self.capacity = int(capacity_bytes)
         self.max_films = int(max_films)
         self._film_bytes = film_bytes
@@ -404,7 +407,8 @@ def perop_step_seconds(r: Request, steps: int = 20, warm: int = 3) -> float:
     return dt
 
 
-def solve_alone(r: Request, mode: str = "fused", T=np.float32) -> Dict[str, np.ndarray]:
+def solve_alone(r: Request, mode: str = "fused", T=np.float32,
+                final_outputs_only: bool = True) -> Dict[str, np.ndarray]:
     """The request run by itself on the GPU: a single-film state, the same
     initial height, then `mode` = "fused" (one fused kernel per step) or
     "perop" (the reference's op-by-op step: filmpressure!, h∇p!, slippage!,
@@ -427,6 +431,7 @@ def solve_alone(r: Request, mode: str = "fused", T=np.float32) -> Dict[str, np.n
         # the kernel variant batches use, so that the bits agree
         o.thermal, o.seed = int(r.init.thermal), int(r.init.seed)
         o.variant, o.tile_y = BATCH_VARIANT, BATCH_TILE_Y
+        o.final_outputs_only = int(bool(final_outputs_only))
         o.body = int(r.init.body)
         o.body_ax, o.body_ay, o.body_fac = r.init.body_ax, r.init.body_ay, r.init.body_fac
         prm = r.param.cparams()

@@ -116,6 +116,25 @@ def test_batch_matches_each_film_alone_bitwise(S):
     b.close()
 
 
+def test_final_outputs_only_batch_matches_alone(S):
+    """Films admitted with final_outputs_only write distributions and pressure
+    on their last step only: every field at the end is the same bits as the
+    film alone; reading FOUT / PRESSURE of a running τ = 1 film is an error."""
+    specs = film_specs(S)
+    b = S.Batch(1 << 30, 64, final_outputs_only=True)
+    slots = [b.add(prm, Lx, Ly, init, n) for Lx, Ly, prm, init, n in specs]
+    b.step(20)
+    flat = next(i for i, sp in enumerate(specs) if sp[2].τ == 1.0 and sp[4] > 20)
+    with pytest.raises(S.SwalbeError):
+        b.get(slots[flat], "fout")
+    assert b.get(slots[flat], "height").shape == (specs[flat][0], specs[flat][1])
+    b.step(max(s[4] for s in specs))
+    for slot, (Lx, Ly, prm, init, n) in zip(slots, specs):
+        got = {name: b.get(slot, name) for name in FIELDS}
+        assert_same(got, alone(S, Lx, Ly, prm, init, n), f"final-only {Lx}x{Ly}")
+    b.close()
+
+
 def test_staggered_admission_and_slot_reuse(S):
     """Films admitted after others have run, into memory freed by a retired
     film, still give their alone bits; graph replays and single launches mix."""
