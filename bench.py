@@ -68,6 +68,30 @@ def cpu_baseline(seconds_target: float = 12.0):
                       "unfused reference-order sweeps (oracle)"}
 
 
+def cpu_baseline_fused(seconds_target: float = 8.0):
+    """The same step as three fused OpenMP sweeps on all host threads
+    (oracle/swalbe_cpu_fused.c, bit-identical to the oracle), FP64, on a
+    4096 x 512 strip of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    Lx, Ly = 4096, 512
+    st = O.State(Lx, Ly)
+    st.height[...] = O.init_normal(Lx, Ly, 1234, 1.0, 0.001)
+    st.feq[...], _ = O.equilibrium(st.height, st.velx, st.vely, 0.0)
+    prm = O.params(gamma=0.01, kbt=1e-6, tdump=10 ** 9)
+    O.fused_time_loop(st, prm, 2, thermal=True, seed=7)  # threads and pages warm up
+    t = time.perf_counter()
+    O.fused_time_loop(st, prm, 4, thermal=True, seed=7, t0=2)
+    one = (time.perf_counter() - t) / 4
+    n = max(4, int(seconds_target / max(one, 1e-4)))
+    t = time.perf_counter()
+    O.fused_time_loop(st, prm, n, thermal=True, seed=7, t0=6)
+    dt = time.perf_counter() - t
+    return {"value": Lx * Ly * n / dt / 1e9, "unit": "GLUPS", "cores": O.fused_threads(),
+            "kind": "port", "sample": f"{Lx}x{Ly} strip of the same workload, FP64, {n} steps, "
+                                      f"{dt:.1f} s, three fused OpenMP sweeps per step"}
+
+
 def bench_batch(a):
     """256 independent droplet-coalescence films (Lx in {128..1024} step 64,
     Ly = Lx/2, two tangent caps, per-film θ, γ, radius) stepped together by
@@ -524,6 +548,7 @@ def main():
     }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline_fused"] = cpu_baseline_fused()
     print(json.dumps(line), flush=True)
 
 

@@ -648,3 +648,21 @@ def t0(hb=0.07, gamma=0.01, mu=1 / 6, theta=1 / 6):
     out = np.zeros(2)
     lib().orc_t0(C.c_double(hb), C.c_double(gamma), C.c_double(mu), C.c_double(theta), _p(out))
     return out[0], out[1]
+
+
+def fused_time_loop(state: "State", prm: Params, nsteps: int, thermal=False, seed=0, t0=0):
+    """The multi-core CPU baseline (swalbe_cpu_fused.c): the same step as
+    time_loop in three OpenMP sweeps; the same bits."""
+    L = lib()
+    L.orc_fused_time_loop.restype = C.c_int
+    L.orc_fused_time_loop.argtypes = [C.POINTER(StateC), C.POINTER(Params), C.c_int64, C.c_int,
+                                      C.c_uint64, C.c_int64]
+    cs = state.cstruct()
+    rc = L.orc_fused_time_loop(C.byref(cs), C.byref(prm), int(nsteps), int(bool(thermal)),
+                               int(seed), int(t0))
+    if rc != 0:
+        raise MemoryError("orc_fused_time_loop: allocation failed")
+
+
+def fused_threads() -> int:
+    return int(lib().orc_fused_threads())

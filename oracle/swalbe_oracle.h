@@ -106,6 +106,9 @@ int orc_time_loop(orc_state *st, const orc_params *p, int64_t nsteps, const orc_
 
 /* ---- one dimension: D1Q3 (State_1D, src/initialize.jl:276-290); fields of L
  * nodes, distributions (L, 3), population k at offset k*L ---- */
+int orc_fused_time_loop(orc_state *s, const orc_params *p, int64_t nsteps, int thermal,
+                        uint64_t seed, int64_t t0);
+int orc_fused_threads(void);
 void orc_grad_simple(double *outx, double *outy, const double *f, int Lx, int Ly);
 void orc_measure(const double *h, const double *theta_field, double theta, int Lx, int Ly,
                  double thresh, double out[4], unsigned char *mask);
