@@ -46,6 +46,17 @@ def bytes_per_lup(esz: int, tau_is_one: bool) -> int:
     return (reads + writes) * esz
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(seconds_target: float = 12.0):
     """The oracle's unfused FP64 step (reference-equivalent sweeps, 1 thread)
     timed on a 4096 x 128 strip of the same workload."""
@@ -65,7 +76,7 @@ def cpu_baseline(seconds_target: float = 12.0):
     dt = time.perf_counter() - t
     return {"value": Lx * Ly * n / dt / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
             "sample": f"{Lx}x{Ly} strip of the same workload, FP64, {n} steps, {dt:.1f} s, "
-                      "unfused reference-order sweeps (oracle)"}
+                      f"unfused reference-order sweeps (oracle), {cpu_model()}"}
 
 
 def cpu_baseline_fused(seconds_target: float = 8.0):
@@ -89,7 +100,7 @@ def cpu_baseline_fused(seconds_target: float = 8.0):
     dt = time.perf_counter() - t
     return {"value": Lx * Ly * n / dt / 1e9, "unit": "GLUPS", "cores": O.fused_threads(),
             "kind": "port", "sample": f"{Lx}x{Ly} strip of the same workload, FP64, {n} steps, "
-                                      f"{dt:.1f} s, three fused OpenMP sweeps per step"}
+                                      f"{dt:.1f} s, three fused OpenMP sweeps per step, {cpu_model()}"}
 
 
 def bench_batch(a):
@@ -334,6 +345,29 @@ def bench_copy(a):
                                  "bytes_per_copy": nbytes}}), flush=True)
 
 
+def ceilings(st, tile_y):
+    """What the part allows for this step, measured live beside it: a plain
+    device-to-device copy of nine 4096^2 FP32 planes (GB/s, read + write) and
+    the step's own access pattern with no arithmetic (read 3 / write 13 planes
+    over the same 64 x tile_y tiles, GLUPS)."""
+    import ctypes as C
+    from swalbe import _lib
+    for _ in range(3):
+        st.copy("ftemp", "feq")
+    st.synchronize()
+    reps = 20
+    t = time.perf_counter()
+    for _ in range(reps):
+        st.copy("ftemp", "feq")
+    st.synchronize()
+    copy_gbs = 2 * 9 * 4096 * 4096 * 4 * reps / (time.perf_counter() - t) / 1e9
+    ms = C.c_double()
+    _lib.check(_lib.lib().swalbe_pattern_probe(4096, 4096, 3, 13, tile_y, 50, C.byref(ms)), "probe")
+    return {"copy_GBs": round(copy_gbs, 1),
+            "pattern_GLUPS": round(4096 * 4096 / (ms.value / 1e3) / 1e9, 3),
+            "pattern": f"read 3 / write 13 planes over 64x{tile_y} tiles, no arithmetic"}
+
+
 def bench_pattern(a):
     """The fused step's data movement alone at 4096^2 FP32: read 3 planes (h,
     ux, uy), write 13 (9 f, h, ux, uy, p) over the step's 64 x 16 tiles, no
@@ -520,6 +554,10 @@ def main():
     o.final_outputs_only = 0
     h = st.get("height", np.float64)
     ok = bool(np.isfinite(h).all())
+    ceil = ceilings(st, a.tile_y or 16) if (Lx, Ly, dt) == (4096, 4096, "f32") else None
+    if ceil:
+        ceil["frac_of_copy"] = round(achieved / ceil["copy_GBs"], 4)
+        ceil["frac_of_pattern"] = round(glups / ceil["pattern_GLUPS"], 4)
     line = {
         "metric": "GLUPS (lattice updates/s) of the fused FP32 D2Q9 thin-film step at 4096^2",
         "value": round(glups, 3),
@@ -545,6 +583,7 @@ def main():
         "final_outputs_only": {"GLUPS": round(N * a.steps / (ms2.value / 1e3) / 1e9, 3),
                                "bytes_per_lup": 7 * esz,
                                "note": "f and p written on the last step only (opts)"},
+        "ceilings": ceil,
     }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()

@@ -435,7 +435,7 @@ int swalbe_batch_add(swalbe_batch* b, const swalbe_params* p, int32_t Lx, int32_
   } else {
     d.ntx = (Lx + 63) / 64;
     d.ntiles = d.ntx * ((Ly + kBatchTileY - 1) / kBatchTileY);
-    d.small = film_is_small(Lx, Ly, kBatchTileY);
+    d.small = film_is_small(Lx, Ly, 64, kBatchTileY);
     d.pairs = (!d.small && (Ly % 2) == 0) ? 1 : 0;
   }
   d.read_ft = two_f ? 1 : 0;

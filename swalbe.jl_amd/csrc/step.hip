@@ -126,22 +126,76 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   const int tid = threadIdx.x;
   const int64_t N = (int64_t)Lx * Ly;
 
-  // ---- phase A: stage h (3-halo) and u (1-halo)
-  for (int idx = tid; idx < HR * HC; idx += NT) {
-    const int r = idx / HC, c = idx - r * HC;
-    int gx = x0 - 3 + c, gy = y0 - 3 + r;
-    gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
-    gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
-    sh_h[idx] = io.h_in[gx + (int64_t)Lx * gy];
-  }
-  for (int idx = tid; idx < UR * UC; idx += NT) {
-    const int r = idx / UC, c = idx - r * UC;
-    int gx = x0 - 1 + c, gy = y0 - 1 + r;
-    gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
-    gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
-    const int64_t g = gx + (int64_t)Lx * gy;
-    sh_ux[idx] = io.ux_in[g];
-    sh_uy[idx] = io.uy_in[g];
+  // ---- phase A: stage h (3-halo) and u (1-halo).  Every load is issued
+  // into registers before the first LDS store (memory-level parallelism);
+  // interior tiles (no wrap) address by 32-bit byte offsets from the halo's
+  // corner.
+  {
+    constexpr int HN = HR * HC, UN = UR * UC;
+    constexpr int NHI = (HN + NT - 1) / NT, NUI = (UN + NT - 1) / NT;
+    T hreg[NHI], ureg[NUI], vreg[NUI];
+    if (!small && x0 >= 3 && y0 >= 3 && x0 + TX + 3 <= Lx && y0 + TY + 3 <= Ly &&
+        Lx <= (1 << 23)) {
+      const char* hb = (const char*)(io.h_in + ((x0 - 3) + (int64_t)Lx * (y0 - 3)));
+      const char* ub = (const char*)(io.ux_in + ((x0 - 1) + (int64_t)Lx * (y0 - 1)));
+      const char* vb = (const char*)(io.uy_in + ((x0 - 1) + (int64_t)Lx * (y0 - 1)));
+#pragma unroll
+      for (int i = 0; i < NHI; ++i) {
+        const int e = tid + i * NT;
+        if (e < HN) {
+          const int r = e / HC, c = e - r * HC;
+          hreg[i] = *(const T*)(hb + (uint32_t)((int)sizeof(T) * (c + Lx * r)));
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NUI; ++i) {
+        const int e = tid + i * NT;
+        if (e < UN) {
+          const int r = e / UC, c = e - r * UC;
+          const uint32_t o = (uint32_t)((int)sizeof(T) * (c + Lx * r));
+          ureg[i] = *(const T*)(ub + o);
+          vreg[i] = *(const T*)(vb + o);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NHI; ++i) {
+        const int e = tid + i * NT;
+        if (e < HN) {
+          const int r = e / HC, c = e - r * HC;
+          int gx = x0 - 3 + c, gy = y0 - 3 + r;
+          gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
+          gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
+          hreg[i] = io.h_in[gx + (int64_t)Lx * gy];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NUI; ++i) {
+        const int e = tid + i * NT;
+        if (e < UN) {
+          const int r = e / UC, c = e - r * UC;
+          int gx = x0 - 1 + c, gy = y0 - 1 + r;
+          gx = small ? wrapm(gx, Lx) : wrap1(gx, Lx);
+          gy = small ? wrapm(gy, Ly) : wrap1(gy, Ly);
+          const int64_t g = gx + (int64_t)Lx * gy;
+          ureg[i] = io.ux_in[g];
+          vreg[i] = io.uy_in[g];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NHI; ++i) {
+      const int e = tid + i * NT;
+      if (e < HN) sh_h[e] = hreg[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NUI; ++i) {
+      const int e = tid + i * NT;
+      if (e < UN) {
+        sh_ux[e] = ureg[i];
+        sh_uy[e] = vreg[i];
+      }
+    }
   }
   __syncthreads();
 

@@ -54,11 +54,11 @@ __device__ __forceinline__ void store_f(float* p, float v, int nts) {
   else *p = v;
 }
 
-template <int TY, bool SRC>
+template <int TXW, int TY, bool SRC>
 struct Smem {
-  static constexpr int HC = TX + 6, HR = TY + 6, HN = HR * HC;
-  static constexpr int PC = TX + 4, PR = TY + 4, PN = PR * PC;
-  static constexpr int UC = TX + 2, UR = TY + 2, UN = UR * UC;
+  static constexpr int HC = TXW + 6, HR = TY + 6, HN = HR * HC;
+  static constexpr int PC = TXW + 4, PR = TY + 4, PN = PR * PC;
+  static constexpr int UC = TXW + 2, UR = TY + 2, UN = UR * UC;
   float h[HN];
   float p[PN];
   float u[UN], v[UN];
@@ -90,15 +90,108 @@ struct SeriesAcc {
   unsigned int cnt;
 };
 
-// One 64 x TY tile of one film: stage the halo, then pressure, forces,
-// collision, pull-stream and moments.  The caller synchronises the block
-// between two calls (the tile rewrites the LDS halo first).
-template <int TY, bool READ_FT, bool THERMAL, bool SRC, bool SERIES>
-__device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const TileRun& R,
-                                          int x0, int y0, Smem<TY, SRC>& sm, SeriesAcc& acc) {
+// Halo of one tile in registers: h on the 3-halo and (ux, uy) on the 1-halo,
+// flat over the block (element e = tid + i * NTH).
+template <int TXW, int NTH, int TY>
+struct Halo {
+  using SM = Smem<TXW, TY, false>;
+  static constexpr int NHI = (SM::HN + NTH - 1) / NTH, NUI = (SM::UN + NTH - 1) / NTH;
+  float h[NHI], u[NUI], v[NUI];
+};
+
+// ---- A1: issue every halo load of tile (x0, y0) (memory-level parallelism:
+// nothing waits for them until halo_commit)
+template <int TXW, int NTH, int TY>
+__device__ __forceinline__ void halo_load(const TileIO& io, const TileRun& R, int x0, int y0,
+                                          Halo<TXW, NTH, TY>& hr) {
+  using SM = Smem<TXW, TY, false>;
+  using HR = Halo<TXW, NTH, TY>;
+  constexpr int HC = SM::HC, HN = SM::HN, UC = SM::UC, UN = SM::UN;
+  const int tid = threadIdx.x;
+  const int Lx = R.Lx, Ly = R.Ly;
+  const int small = R.small;
+  // Interior tiles (the 3-halo wraps nowhere): 32-bit byte offsets from the
+  // halo's corner, so each load is a uniform base plus one VGPR offset
+  if (!small && x0 >= 3 && y0 >= 3 && x0 + TXW + 3 <= Lx && y0 + TY + 3 <= Ly &&
+      Lx <= (1 << 24)) {
+    const char* hb = (const char*)(io.h_in + ((x0 - 3) + (int64_t)Lx * (y0 - 3)));
+    const char* ub = (const char*)(io.ux_in + ((x0 - 1) + (int64_t)Lx * (y0 - 1)));
+    const char* vb = (const char*)(io.uy_in + ((x0 - 1) + (int64_t)Lx * (y0 - 1)));
+#pragma unroll
+    for (int i = 0; i < HR::NHI; ++i) {
+      const int e = tid + i * NTH;
+      if (e < HN) {
+        const int r = e / HC, c = e - r * HC;
+        hr.h[i] = *(const float*)(hb + (uint32_t)(4 * (c + Lx * r)));
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < HR::NUI; ++i) {
+      const int e = tid + i * NTH;
+      if (e < UN) {
+        const int r = e / UC, c = e - r * UC;
+        const uint32_t o = (uint32_t)(4 * (c + Lx * r));
+        hr.u[i] = *(const float*)(ub + o);
+        hr.v[i] = *(const float*)(vb + o);
+      }
+    }
+    return;
+  }
+  auto wx = [&](int a) { return small ? wrapm(a, Lx) : wrap1(a, Lx); };
+  auto wy = [&](int a) { return small ? wrapm(a, Ly) : wrap1(a, Ly); };
+#pragma unroll
+  for (int i = 0; i < HR::NHI; ++i) {
+    const int e = tid + i * NTH;
+    if (e < HN) {
+      const int r = e / HC, c = e - r * HC;
+      hr.h[i] = io.h_in[wx(x0 - 3 + c) + (int64_t)Lx * wy(y0 - 3 + r)];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < HR::NUI; ++i) {
+    const int e = tid + i * NTH;
+    if (e < UN) {
+      const int r = e / UC, c = e - r * UC;
+      const int64_t g = wx(x0 - 1 + c) + (int64_t)Lx * wy(y0 - 1 + r);
+      hr.u[i] = io.ux_in[g];
+      hr.v[i] = io.uy_in[g];
+    }
+  }
+}
+
+// ---- A2: park the halo in LDS (the block synchronises before phase B)
+template <int TXW, int NTH, int TY, bool SRC>
+__device__ __forceinline__ void halo_commit(const Halo<TXW, NTH, TY>& hr,
+                                            Smem<TXW, TY, SRC>& sm) {
+  using SM = Smem<TXW, TY, SRC>;
+  using HR = Halo<TXW, NTH, TY>;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < HR::NHI; ++i) {
+    const int e = tid + i * NTH;
+    if (e < SM::HN) sm.h[e] = hr.h[i];
+  }
+#pragma unroll
+  for (int i = 0; i < HR::NUI; ++i) {
+    const int e = tid + i * NTH;
+    if (e < SM::UN) {
+      sm.u[e] = hr.u[i];
+      sm.v[e] = hr.v[i];
+    }
+  }
+}
+
+// Phases B-D of one TXW x TY tile whose halo is in LDS: pressure, forces,
+// collision, pull-stream and moments, computed by NTH threads.
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool SERIES>
+__device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, const TileRun& R,
+                                             int x0, int y0, Smem<TXW, TY, SRC>& sm,
+                                             SeriesAcc& acc) {
 #pragma clang fp contract(fast)
-  using SM = Smem<TY, SRC>;
-  constexpr int HC = SM::HC, HR = SM::HR;
+  static_assert(TXW % 64 == 0 && NTH % 64 == 0, "tiles are whole wavefronts wide");
+  constexpr int NT = NTH, NW = NTH / 64;
+  using SM = Smem<TXW, TY, SRC>;
+  constexpr int HC = SM::HC;
   constexpr int PC = SM::PC, PN = SM::PN;
   constexpr int UC = SM::UC, UR = SM::UR, UN = SM::UN;
   const int tid = threadIdx.x;
@@ -110,46 +203,6 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
   const int small = R.small;
   auto wx = [&](int a) { return small ? wrapm(a, Lx) : wrap1(a, Lx); };
   auto wy = [&](int a) { return small ? wrapm(a, Ly) : wrap1(a, Ly); };
-
-  // ---- A: h on the 3-halo and u on the 1-halo, flat over the block; all
-  // loads are issued before the first LDS store (memory-level parallelism)
-  {
-    constexpr int HN = SM::HN;
-    constexpr int NHI = (HN + NT - 1) / NT, NUI = (UN + NT - 1) / NT;
-    float hreg[NHI], ureg[NUI], vreg[NUI];
-#pragma unroll
-    for (int i = 0; i < NHI; ++i) {
-      const int e = tid + i * NT;
-      if (e < HN) {
-        const int r = e / HC, c = e - r * HC;
-        hreg[i] = io.h_in[wx(x0 - 3 + c) + (int64_t)Lx * wy(y0 - 3 + r)];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NUI; ++i) {
-      const int e = tid + i * NT;
-      if (e < UN) {
-        const int r = e / UC, c = e - r * UC;
-        const int64_t g = wx(x0 - 1 + c) + (int64_t)Lx * wy(y0 - 1 + r);
-        ureg[i] = io.ux_in[g];
-        vreg[i] = io.uy_in[g];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NHI; ++i) {
-      const int e = tid + i * NT;
-      if (e < HN) sm.h[e] = hreg[i];
-    }
-#pragma unroll
-    for (int i = 0; i < NUI; ++i) {
-      const int e = tid + i * NT;
-      if (e < UN) {
-        sm.u[e] = ureg[i];
-        sm.v[e] = vreg[i];
-      }
-    }
-  }
-  __syncthreads();
 
   // ---- B: film pressure on the 2-halo
   for (int e = tid; e < PN; e += NT) {
@@ -269,21 +322,22 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
   }
   __syncthreads();
 
-  // ---- D: stream (gather) and moments
-  const int gxo = x0 + lane;
-  for (int row = wv; row < TY; row += NW) {
+  // ---- D: stream (gather) and moments; a wave takes whole 64-column rows
+  for (int rc = wv; rc < TY * (TXW / 64); rc += NW) {
+    const int row = rc / (TXW / 64), col = lane + 64 * (rc - row * (TXW / 64));
+    const int gxo = x0 + col;
     const int gyo = y0 + row;
     if (gxo >= Lx || gyo >= Ly) continue;
     const int64_t xo = gxo + (int64_t)Lx * gyo;
     float fo[kQ];
 #pragma unroll
     for (int k = 0; k < kQ; ++k) {
-      const int su = (row + 1 - cy(k)) * UC + (lane + 1 - cx(k));
+      const int su = (row + 1 - cy(k)) * UC + (col + 1 - cx(k));
       float f;
       if constexpr (SRC) {
         f = sm.a[k * UN + su];
       } else {
-        const float dh = sm.h[(row + 3 - cy(k)) * HC + (lane + 3 - cx(k))];
+        const float dh = sm.h[(row + 3 - cy(k)) * HC + (col + 3 - cx(k))];
         const float hh = K.href + dh;
         const float u = sm.u[su], v = sm.v[su];
         const float usq = u * u + v * v;
@@ -317,14 +371,29 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
     store_f(&io.h_out[xo], s, R.nts);
     store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, R.nts);
     store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, R.nts);
-    if (io.p_out) store_f(&io.p_out[xo], sm.p[(row + 2) * PC + (lane + 2)], R.nts);
+    if (io.p_out) store_f(&io.p_out[xo], sm.p[(row + 2) * PC + (col + 2)], R.nts);
     if constexpr (SERIES) {
-      const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (lane + 3)];
+      const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (col + 3)];
       acc.lmax = hv > acc.lmax ? hv : acc.lmax;
       acc.lmin = hv < acc.lmin ? hv : acc.lmin;
       acc.cnt += hv > R.wet_thresh ? 1u : 0u;
     }
   }
+}
+
+
+// One TXW x TY tile of one film: stage the halo, then phases B-D.  The caller
+// synchronises the block between two calls (the tile rewrites the LDS halo
+// first).
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool SERIES>
+__device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const TileRun& R,
+                                          int x0, int y0, Smem<TXW, TY, SRC>& sm,
+                                          SeriesAcc& acc) {
+  Halo<TXW, NTH, TY> hr;
+  halo_load<TXW, NTH, TY>(io, R, x0, y0, hr);
+  halo_commit<TXW, NTH, TY, SRC>(hr, sm);
+  __syncthreads();
+  tile_compute<TXW, NTH, TY, READ_FT, THERMAL, SRC, SERIES>(io, K, R, x0, y0, sm, acc);
 }
 
 
@@ -567,23 +636,11 @@ __global__ __launch_bounds__(64) void k_strip_f32(TileIO io, F32K K, TileRun R, 
 }
 
 // ---------------------------------------------------------------- single film
-template <int TY, bool READ_FT, bool THERMAL, bool SRC>
-__global__ __launch_bounds__(NT) void k_step_f32(TileIO io, F32K K, TileRun R, int ntx,
-                                                 int ntiles, StepSeries ser) {
-  __shared__ Smem<TY, SRC> sm;
-  __shared__ unsigned long long red_k[2 * NW];
-  __shared__ unsigned int red_c[NW];
-  // XCD-aware tile order: consecutive block ids of one XCD get neighbouring
-  // tiles (shared halos in that XCD's L2)
-  int tile = blockIdx.x;
-  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
-  const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY;
-  SeriesAcc acc{-__builtin_inf(), __builtin_inf(), 0u};
-  const bool series = ser.dh_keys || ser.wet;
-  if (series) tile_step<TY, READ_FT, THERMAL, SRC, true>(io, K, R, x0, y0, sm, acc);
-  else tile_step<TY, READ_FT, THERMAL, SRC, false>(io, K, R, x0, y0, sm, acc);
-  if (!series) return;
-  // per-step series: max/min/count of h_t over the domain (exact)
+// per-step series: max/min/count of h_t over the block's tiles, folded into
+// the domain totals (exact: integer keys and counts)
+template <int NW>
+__device__ __forceinline__ void series_flush(const SeriesAcc& acc, const StepSeries& ser,
+                                             unsigned long long* red_k, unsigned int* red_c) {
   unsigned long long kmax = dkey(acc.lmax), kmin = dkey(acc.lmin);
   unsigned int lcnt = acc.cnt;
   for (int o = 32; o > 0; o >>= 1) {
@@ -613,16 +670,35 @@ __global__ __launch_bounds__(NT) void k_step_f32(TileIO io, F32K K, TileRun R, i
   }
 }
 
-template <int TY, bool SRC>
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC>
+__global__ __launch_bounds__(NTH) void k_step_f32(TileIO io, F32K K, TileRun R, int ntx,
+                                                  int ntiles, StepSeries ser) {
+  constexpr int NW = NTH / 64;
+  __shared__ Smem<TXW, TY, SRC> sm;
+  __shared__ unsigned long long red_k[2 * NW];
+  __shared__ unsigned int red_c[NW];
+  // XCD-aware tile order: consecutive block ids of one XCD get neighbouring
+  // tiles (shared halos in that XCD's L2)
+  int tile = blockIdx.x;
+  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+  const int x0 = (tile % ntx) * TXW, y0 = (tile / ntx) * TY;
+  SeriesAcc acc{-__builtin_inf(), __builtin_inf(), 0u};
+  const bool series = ser.dh_keys || ser.wet;
+  if (series) tile_step<TXW, NTH, TY, READ_FT, THERMAL, SRC, true>(io, K, R, x0, y0, sm, acc);
+  else tile_step<TXW, NTH, TY, READ_FT, THERMAL, SRC, false>(io, K, R, x0, y0, sm, acc);
+  if (series) series_flush<NW>(acc, ser, red_k, red_c);
+}
+
+template <int TXW, int NTH, int TY, bool SRC>
 void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
                    int Lx, int Ly, int64_t fstride, Enc enc, int nts, hipStream_t s) {
-  const int ntx = (Lx + TX - 1) / TX, nty = (Ly + TY - 1) / TY;
+  const int ntx = (Lx + TXW - 1) / TXW, nty = (Ly + TY - 1) / TY;
   TileIO t{io.h_in, io.ux_in, io.uy_in, io.ft, io.f_out, io.h_out, io.ux_out, io.uy_out,
            io.p_out, io.kappa};
   TileRun R;
   R.Lx = Lx;
   R.Ly = Ly;
-  R.small = film_is_small(Lx, Ly, TY);
+  R.small = film_is_small(Lx, Ly, TXW, TY);
   R.pairs = (!R.small && (Ly % 2) == 0) ? 1 : 0;
   R.nts = nts;
   R.use_kappa = io.kappa != nullptr;
@@ -632,9 +708,10 @@ void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser
   R.step = step;
   R.wet_thresh = sp.wet_thresh;
   const F32K K = make_f32k(sp, enc);
-  const dim3 grid(ntx * nty), block(NT);
-#define SW_L(RF, TH) \
-  hipLaunchKernelGGL((k_step_f32<TY, RF, TH, SRC>), grid, block, 0, s, t, K, R, ntx, ntx * nty, ser)
+  const dim3 grid(ntx * nty), block(NTH);
+#define SW_L(RF, TH)                                                                         \
+  hipLaunchKernelGGL((k_step_f32<TXW, NTH, TY, RF, TH, SRC>), grid, block, 0, s, t, K, R, ntx, \
+                     ntx * nty, ser)
   if (sp.read_ft) {
     if (sp.thermal) SW_L(true, true);
     else SW_L(true, false);
@@ -657,7 +734,7 @@ __global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ 
                                                   const BatchHdr* __restrict__ hdr,
                                                   const unsigned long long* __restrict__ counter,
                                                   int nts) {
-  __shared__ Smem<TY, false> sm;
+  __shared__ Smem<TX, TY, false> sm;
   const int total = hdr->total_tiles, nf = hdr->nfilms;
   const unsigned long long L = *counter;
   // each workgroup takes a contiguous run of its XCD's share: consecutive
@@ -724,11 +801,11 @@ __global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ 
     const int x0 = (local % F.ntx) * TX, y0 = (local / F.ntx) * TY;
     const F32K K = F.K;
     if (F.read_ft) {
-      if (F.thermal) tile_step<TY, true, true, false, false>(io, K, R, x0, y0, sm, acc);
-      else tile_step<TY, true, false, false, false>(io, K, R, x0, y0, sm, acc);
+      if (F.thermal) tile_step<TX, NT, TY, true, true, false, false>(io, K, R, x0, y0, sm, acc);
+      else tile_step<TX, NT, TY, true, false, false, false>(io, K, R, x0, y0, sm, acc);
     } else {
-      if (F.thermal) tile_step<TY, false, true, false, false>(io, K, R, x0, y0, sm, acc);
-      else tile_step<TY, false, false, false, false>(io, K, R, x0, y0, sm, acc);
+      if (F.thermal) tile_step<TX, NT, TY, false, true, false, false>(io, K, R, x0, y0, sm, acc);
+      else tile_step<TX, NT, TY, false, false, false, false>(io, K, R, x0, y0, sm, acc);
     }
     __syncthreads();
   }
@@ -835,12 +912,13 @@ F32K make_f32k(const StepParams& sp, const Enc& enc) {
   return K;
 }
 
-int film_is_small(int Lx, int Ly, int tile_y) {
-  return (Lx < TX + 6 || Ly < tile_y + 6) ? 1 : 0;
+int film_is_small(int Lx, int Ly, int tile_x, int tile_y) {
+  return (Lx < tile_x + 6 || Ly < tile_y + 6) ? 1 : 0;
 }
 
 // variant: 0 = default (collide at the destination), 1 = collide at the
-// source node, 2 = collide at the destination.  nts: non-temporal stores.
+// source node, 2 = collide at the destination, 3 = LDS-free strips.
+// nts: non-temporal stores.
 void launch_strip(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
                   int Lx, int Ly, int64_t fstride, Enc enc, int seg, int nts, hipStream_t s) {
   TileIO t{io.h_in, io.ux_in, io.uy_in, io.ft, io.f_out, io.h_out, io.ux_out, io.uy_out,
@@ -883,11 +961,11 @@ void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries s
   if (variant == 3) {
     launch_strip(io, sp, ser, step, Lx, Ly, fstride, enc, tile_y > 0 ? tile_y : kStripSeg, nts, s);
   } else if (variant == 1) {
-    if (t8) launch_single<8, true>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
-    else launch_single<16, true>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
+    if (t8) launch_single<TX, NT, 8, true>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
+    else launch_single<TX, NT, 16, true>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
   } else {
-    if (t8) launch_single<8, false>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
-    else launch_single<16, false>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
+    if (t8) launch_single<TX, NT, 8, false>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
+    else launch_single<TX, NT, 16, false>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s);
   }
 }
 

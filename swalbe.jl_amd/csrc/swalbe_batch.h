@@ -14,7 +14,7 @@ struct F32K {
 };
 
 F32K make_f32k(const StepParams& sp, const Enc& enc);
-int film_is_small(int Lx, int Ly, int tile_y);
+int film_is_small(int Lx, int Ly, int tile_x, int tile_y);
 
 // Rows per tile of the batched kernel.
 constexpr int kBatchTileY = 16;
