@@ -294,7 +294,7 @@ def bench_long1d(a):
     S.equilibrium(st, sysc)
     prm = sysc.param.cparams()
     o = _lib.StepOpts1D()
-    o.variant, o.materialize = 1, 0
+    o.variant, o.materialize, o.threads = 1, 0, a.threads
     ms = C.c_double()
     o.t0 = 0
     _lib.check(_lib.lib().swalbe1d_time_loop_timed(st.handle, C.byref(prm), a.warmup, C.byref(o),

@@ -302,28 +302,31 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 // One workgroup: output nodes x0 .. x0+255; h on a 3-halo, u and F on 1, p on 2.
-template <typename T>
-__global__ __launch_bounds__(kNT1) void k1_step(Film1 S, uint64_t step,
-                                                unsigned long long* dh_keys,
-                                                unsigned long long* wet) {
-  __shared__ T sh[kNT1 + 6], su[kNT1 + 2], sp[kNT1 + 4], sF[kNT1 + 2];
+template <typename T, int NB>
+__global__ __launch_bounds__(NB) void k1_step(Film1 S, uint64_t step,
+                                              unsigned long long* dh_keys,
+                                              unsigned long long* wet) {
+  __shared__ T sh[NB + 6], su[NB + 2], sp[NB + 4], sF[NB + 2];
   const int t = threadIdx.x, L = S.L;
-  const int x0 = blockIdx.x * kNT1;
+  const int x0 = blockIdx.x * NB;
   const T* h_in = (const T*)S.h_in;
   const T* u_in = (const T*)S.u_in;
   const T* kap_f = (const T*)S.kap;
   const T* gam_f = (const T*)S.gam;
-  for (int i = t; i < kNT1 + 6; i += kNT1) sh[i] = h_in[wrapL(x0 - 3 + i, L)];
-  for (int i = t; i < kNT1 + 2; i += kNT1) su[i] = u_in[wrapL(x0 - 1 + i, L)];
+  // a workgroup whose 3-halo wraps nowhere indexes directly (no modulo)
+  const bool inner = x0 >= 3 && x0 + NB + 3 <= L;
+  auto wr = [&](int y) { return inner ? y : wrapL(y, L); };
+  for (int i = t; i < NB + 6; i += NB) sh[i] = h_in[wr(x0 - 3 + i)];
+  for (int i = t; i < NB + 2; i += NB) su[i] = u_in[wr(x0 - 1 + i)];
   __syncthreads();
-  for (int i = t; i < kNT1 + 4; i += kNT1) {  // p at node x0 - 2 + i
-    const int y = wrapL(x0 - 2 + i, L);
+  for (int i = t; i < NB + 4; i += NB) {  // p at node x0 - 2 + i
+    const int y = wr(x0 - 2 + i);
     sp[i] = pressure_node(sh[i], sh[i + 1], sh[i + 2], gam_f ? gam_f[y] : (T)S.gamma,
                           kap_f ? kap_f[y] : (T)S.kap_s, S.n, S.m, (T)S.hmin, (T)S.hcrit, 0);
   }
   __syncthreads();
-  for (int i = t; i < kNT1 + 2; i += kNT1) {  // F at node x0 - 1 + i
-    const int y = wrapL(x0 - 1 + i, L);
+  for (int i = t; i < NB + 2; i += NB) {  // F at node x0 - 1 + i
+    const int y = wr(x0 - 1 + i);
     const T hh = sh[i + 2];
     const T hgp = (hh * T(-0.5)) * (sp[i] - sp[i + 2]);
     T F = -hgp - slip_node(hh, su[i], (T)S.c6mu, (T)S.c6d, (T)S.c3dd);
@@ -1245,7 +1248,9 @@ int loop_per_step(swalbe_film1d* s, const swalbe_params* p, int64_t nsteps,
     Film1 S = describe(s, p, o);
     int a = s->ftemp_buf, c = s->cur;
     int64_t km = 0, ks = 0;
-    const int nb = (s->L + kNT1 - 1) / kNT1;
+    // nodes (threads) per workgroup of k1_step: opts.threads, else 256
+    const int nbt = o->threads ? o->threads : 256;
+    const int nb = (s->L + nbt - 1) / nbt;
     for (int64_t it = 1; it <= nsteps; ++it) {
       const int64_t t = o->t0 + it;
       S.h_in = s->hbuf[c];
@@ -1260,9 +1265,17 @@ int loop_per_step(swalbe_film1d* s, const swalbe_params* p, int64_t nsteps,
                                        mass + 4 * km));
         ++km;
       }
-      SW_BOTH1(s, hipLaunchKernelGGL(k1_step<T>, dim3(nb), dim3(kNT1), 0, s->stream, S,
-                                     (uint64_t)t, keys ? keys + 2 * (it - 1) : nullptr,
-                                     wet ? wet + (it - 1) : nullptr));
+      unsigned long long* kk = keys ? keys + 2 * (it - 1) : nullptr;
+      unsigned long long* ww = wet ? wet + (it - 1) : nullptr;
+      if (nbt == 1024)
+        SW_BOTH1(s, hipLaunchKernelGGL((k1_step<T, 1024>), dim3(nb), dim3(1024), 0, s->stream, S,
+                                       (uint64_t)t, kk, ww));
+      else if (nbt == 512)
+        SW_BOTH1(s, hipLaunchKernelGGL((k1_step<T, 512>), dim3(nb), dim3(512), 0, s->stream, S,
+                                       (uint64_t)t, kk, ww));
+      else
+        SW_BOTH1(s, hipLaunchKernelGGL((k1_step<T, 256>), dim3(nb), dim3(256), 0, s->stream, S,
+                                       (uint64_t)t, kk, ww));
       if (snap && t % o->snap_every == 0) {
         (void)hipMemcpyAsync((char*)snap + ks * s->L * s->esz, S.h_out, s->L * s->esz,
                        hipMemcpyDeviceToDevice, s->stream);

@@ -299,9 +299,11 @@ def _options(sys, state, thermal, seed, materialize, variant, grad_gamma_sign):
 def time_loop(sys, state: HipState1D, *args, verbose: bool = False, thermal: bool = False,
               seed: int = 0, materialize: bool = True, variant: int = 0, γ=None,
               grad_gamma_sign: int = 0, mass: Optional[list] = None,
-              snapshots: Optional[np.ndarray] = None, dumping: int = 0):
+              snapshots: Optional[np.ndarray] = None, dumping: int = 0, threads: int = 0):
     """time_loop(sys::SysConst_1D, state[, θ | Δh | f, measure]) and the walled
-    loop (src/simulate.jl:98-205), one fused GPU loop.
+    loop (src/simulate.jl:98-205), one fused GPU loop.  `threads` (0, 256,
+    512, 1024): threads of the resident kernel's film, or nodes per workgroup
+    of the one-launch-per-step kernel; 0 picks automatically.
 
     Known callbacks run fused: wetted (count per step) and inclination (α,
     body force).  Another callable f(measure, state) runs the per-op kernels
@@ -313,6 +315,7 @@ def time_loop(sys, state: HipState1D, *args, verbose: bool = False, thermal: boo
     prm = P.cparams()
     nsteps = P.Tmax
     o = _options(sys, state, thermal, seed, materialize, variant, grad_gamma_sign)
+    o.threads = int(threads)
     if γ is not None:
         if np.ndim(γ) > 0:
             state.set("gamma", γ)

@@ -236,6 +236,28 @@ def test_fp64_time_loop_bitwise_vs_oracle(S, name, variant):
         assert np.array_equal(np.array(dh), out["dh"])
 
 
+@pytest.mark.parametrize("threads", [512, 1024])
+@pytest.mark.parametrize("L,name", [(70, "thermal"), (3001, "thermal"), (1500, "walls"),
+                                    (2049, "gamma_minus"), (1100, "theta_field")],
+                         ids=lambda v: str(v))
+def test_per_step_workgroup_sizes_bitwise_vs_oracle(S, L, name, threads):
+    """The one-launch-per-step kernel with 512 or 1024 nodes per workgroup
+    (a film shorter than one workgroup, ragged lengths, walls, ∇γ, θ field):
+    the oracle's bits, Δh series included."""
+    cfg = CONFIGS[name]
+    n = 120
+    print(f"1D per-step {name} L={L} threads {threads}", flush=True)
+    sysc, prm, st, ost, okw, hkw, args = _setup(S, L, n, cfg, variant=1)
+    hkw["threads"] = threads
+    out = O.time_loop_1d(ost, prm, n, **okw)
+    dh = _run_hip(S, sysc, st, hkw, args, cfg.get("body"))
+    for f_h, f_o in (("height", "height"), ("vel", "vel"), ("fout", "fout"),
+                     ("pressure", "pressure")):
+        assert np.array_equal(st.get(f_h), getattr(ost, f_o)), (name, f_h)
+    if isinstance(dh, list) and dh:
+        assert np.array_equal(np.array(dh), out["dh"])
+
+
 @pytest.mark.parametrize("L", [3, 255, 257, 1000])
 def test_fp64_lengths_and_both_loops_agree(S, L):
     print(f"1D lengths L={L}", flush=True)
