@@ -671,12 +671,11 @@ __device__ __forceinline__ void series_flush(const SeriesAcc& acc, const StepSer
 }
 
 template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC>
-__global__ __launch_bounds__(NTH) void k_step_f32(TileIO io, F32K K, TileRun R, int ntx,
-                                                  int ntiles, StepSeries ser) {
+__device__ __forceinline__ void single_film_step(const TileIO& io, const F32K& K, const TileRun& R,
+                                                 int ntx, int ntiles, const StepSeries& ser,
+                                                 Smem<TXW, TY, SRC>& sm,
+                                                 unsigned long long* red_k, unsigned int* red_c) {
   constexpr int NW = NTH / 64;
-  __shared__ Smem<TXW, TY, SRC> sm;
-  __shared__ unsigned long long red_k[2 * NW];
-  __shared__ unsigned int red_c[NW];
   // XCD-aware tile order: consecutive block ids of one XCD get neighbouring
   // tiles (shared halos in that XCD's L2)
   int tile = blockIdx.x;
@@ -687,6 +686,30 @@ __global__ __launch_bounds__(NTH) void k_step_f32(TileIO io, F32K K, TileRun R, 
   if (series) tile_step<TXW, NTH, TY, READ_FT, THERMAL, SRC, true>(io, K, R, x0, y0, sm, acc);
   else tile_step<TXW, NTH, TY, READ_FT, THERMAL, SRC, false>(io, K, R, x0, y0, sm, acc);
   if (series) series_flush<NW>(acc, ser, red_k, red_c);
+}
+
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC>
+__global__ __launch_bounds__(NTH) void k_step_f32(TileIO io, F32K K, TileRun R, int ntx,
+                                                  int ntiles, StepSeries ser) {
+  __shared__ Smem<TXW, TY, SRC> sm;
+  __shared__ unsigned long long red_k[2 * (NTH / 64)];
+  __shared__ unsigned int red_c[NTH / 64];
+  single_film_step<TXW, NTH, TY, READ_FT, THERMAL, SRC>(io, K, R, ntx, ntiles, ser, sm, red_k,
+                                                        red_c);
+}
+
+// The same kernel under its own name, launched for the steps whose
+// distributions and pressure are not written (f_out = p_out = null): the
+// identical body with the same runtime checks, so both give the same bits,
+// and a kernel trace tells the two kinds of step apart.
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC>
+__global__ __launch_bounds__(NTH) void k_step_f32_unwritten(TileIO io, F32K K, TileRun R,
+                                                            int ntx, int ntiles, StepSeries ser) {
+  __shared__ Smem<TXW, TY, SRC> sm;
+  __shared__ unsigned long long red_k[2 * (NTH / 64)];
+  __shared__ unsigned int red_c[NTH / 64];
+  single_film_step<TXW, NTH, TY, READ_FT, THERMAL, SRC>(io, K, R, ntx, ntiles, ser, sm, red_k,
+                                                        red_c);
 }
 
 template <int TXW, int NTH, int TY, bool SRC>
@@ -709,15 +732,18 @@ void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser
   R.wet_thresh = sp.wet_thresh;
   const F32K K = make_f32k(sp, enc);
   const dim3 grid(ntx * nty), block(NTH);
-#define SW_L(RF, TH)                                                                         \
-  hipLaunchKernelGGL((k_step_f32<TXW, NTH, TY, RF, TH, SRC>), grid, block, 0, s, t, K, R, ntx, \
-                     ntx * nty, ser)
+#define SW_L(KN, RF, TH)                                                                     \
+  hipLaunchKernelGGL((KN<TXW, NTH, TY, RF, TH, SRC>), grid, block, 0, s, t, K, R, ntx, ntx * nty, \
+                     ser)
   if (sp.read_ft) {
-    if (sp.thermal) SW_L(true, true);
-    else SW_L(true, false);
+    if (sp.thermal) SW_L(k_step_f32, true, true);
+    else SW_L(k_step_f32, true, false);
+  } else if (!io.f_out && !io.p_out) {  // a step whose f and p nobody reads
+    if (sp.thermal) SW_L(k_step_f32_unwritten, false, true);
+    else SW_L(k_step_f32_unwritten, false, false);
   } else {
-    if (sp.thermal) SW_L(false, true);
-    else SW_L(false, false);
+    if (sp.thermal) SW_L(k_step_f32, false, true);
+    else SW_L(k_step_f32, false, false);
   }
 #undef SW_L
 }
