@@ -482,7 +482,8 @@ def main():
                     help="1D resident kernel: threads per film (0 = by L, 256, 512, 1024)")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0)
-    ap.add_argument("--nontemporal", type=int, default=0)
+    ap.add_argument("--nontemporal", type=int, default=2,
+                    help="FP32 stores: 0 cached, 1 all streaming, 2 f and p streaming")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

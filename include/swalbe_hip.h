@@ -159,7 +159,10 @@ typedef struct swalbe_step_opts {
     int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = LDS tiles, collide at the
                               source node, 2 = LDS tiles, collide at the destination node,
                               3 = LDS-free strips (one wavefront per 58-column strip) */
-    int32_t nontemporal;   /* FP32 kernel: streaming (non-temporal) stores of the outputs */
+    int32_t nontemporal;   /* FP32 kernel stores (results are the same bits either way):
+                              0 = cached, 1 = every output streams past the caches
+                              (non-temporal), 2 = the distributions and the pressure
+                              stream while h, ux, uy (the next step's input) stay cached */
     int32_t pad_;
     /* snapshot!(snap, height, t; dumping) (src/measures.jl:99-105): after every
      * step t with t % snap_every == 0 the height is copied to snap_series
@@ -254,6 +257,7 @@ int swalbe_batch_step(swalbe_batch *b, int64_t nsteps);
 int swalbe_batch_step_timed(swalbe_batch *b, int64_t nsteps, double *ms);
 /* replay runs of steps_per_graph steps as one captured HIP graph (0 = off) */
 int swalbe_batch_set_graph(swalbe_batch *b, int32_t steps_per_graph);
+/* stores of the batch's films: 0, 1 or 2 as swalbe_step_opts.nontemporal */
 int swalbe_batch_set_nontemporal(swalbe_batch *b, int32_t on);
 /* Films admitted after this call with τ = 1 write their distributions and
  * pressure on their last step only (their heights and velocities every

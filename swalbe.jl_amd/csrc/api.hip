@@ -927,6 +927,10 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
     set_error("swalbe_time_loop: variant must be 0, 1, 2 or 3");
     return SWALBE_EINVAL;
   }
+  if (o->nontemporal < 0 || o->nontemporal > 2) {
+    set_error("swalbe_time_loop: nontemporal must be 0, 1 or 2");
+    return SWALBE_EINVAL;
+  }
   const bool strips = o->variant == 3;
   const int tile_y = o->tile_y > 0 ? o->tile_y : (strips ? kStripSeg : 16);
   if (strips ? (tile_y < 4 || tile_y > 4096) : (tile_y != 8 && tile_y != 16)) {

@@ -369,8 +369,9 @@ int swalbe_batch_set_graph(swalbe_batch* b, int32_t steps_per_graph) {
 
 int swalbe_batch_set_nontemporal(swalbe_batch* b, int32_t on) {
   if (!b) return bad("null batch");
-  if (b->gexec) swalbe_batch_set_graph(b, b->graph_steps);  // re-capture with the new flag
-  b->nts = on ? 1 : 0;
+  if (on < 0 || on > 2) return bad("nontemporal is 0, 1 or 2");
+  b->nts = on;
+  if (b->gexec) return swalbe_batch_set_graph(b, b->graph_steps);  // re-capture with the new flag
   return SWALBE_OK;
 }
 

@@ -368,9 +368,11 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
     }
     const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) + fo[8];
     const float ih = __builtin_amdgcn_rcpf(K.href + s);
-    store_f(&io.h_out[xo], s, R.nts);
-    store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, R.nts);
-    store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, R.nts);
+    // nts 2: only f and p stream past the caches (h, u are the next step's input)
+    const int ntm = R.nts == 1 ? 1 : 0;
+    store_f(&io.h_out[xo], s, ntm);
+    store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, ntm);
+    store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, ntm);
     if (io.p_out) store_f(&io.p_out[xo], sm.p[(row + 2) * PC + (col + 2)], R.nts);
     if constexpr (SERIES) {
       const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (col + 3)];
@@ -573,9 +575,10 @@ __device__ __forceinline__ void strip_run(const TileIO& io, const F32K& K, const
         const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) +
                         fo[8];
         const float ih = __builtin_amdgcn_rcpf(K.href + s);
-        store_f(&io.h_out[xo], s, R.nts);
-        store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, R.nts);
-        store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, R.nts);
+        const int ntm = R.nts == 1 ? 1 : 0;  // nts 2: h, u stay cacheable
+        store_f(&io.h_out[xo], s, ntm);
+        store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, ntm);
+        store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, ntm);
         if (io.p_out) store_f(&io.p_out[xo], p0.v, R.nts);
       }
     }

@@ -105,6 +105,11 @@ class Batch:
     def set_graph(self, steps: int) -> None:
         check(lib().swalbe_batch_set_graph(self._h, int(steps)), "set_graph")
 
+    def set_nontemporal(self, mode: int) -> None:
+        """Stores of the films (the same bits in every mode): 0 cached,
+        1 every output streaming, 2 distributions and pressure streaming."""
+        check(lib().swalbe_batch_set_nontemporal(self._h, int(mode)), "set_nontemporal")
+
     def wait(self, steps_behind: int = 0) -> None:
         """Block until at most `steps_behind` enqueued steps are pending."""
         check(lib().swalbe_batch_wait(self._h, int(steps_behind)), "wait")

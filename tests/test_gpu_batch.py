@@ -187,6 +187,28 @@ def test_graph_and_plain_launches_agree(S):
             assert np.array_equal(a, ref)
 
 
+def test_nontemporal_modes_give_the_same_bits(S):
+    """The store mode is a cache hint only: every mode gives the same bits,
+    also when it changes between runs of a captured graph (the graph is
+    captured again with the new mode)."""
+    specs = film_specs(S)[:4]
+    outs = []
+    for mode in (0, 1, 2):
+        b = S.Batch(256 << 20, 8, graph_steps=16)
+        slots = [b.add(*_args(sp)) for sp in specs]
+        b.step(40)
+        b.set_nontemporal(mode)
+        b.step(57)
+        outs.append([{k: b.get(s, k) for k in ("height", "velx", "fout")} for s in slots])
+        with pytest.raises(S.SwalbeError):
+            b.set_nontemporal(3)
+        b.close()
+    for o in outs[1:]:
+        for a, ref in zip(o, outs[0]):
+            for k in ref:
+                assert np.array_equal(a[k], ref[k]), k
+
+
 def test_batch_film_vs_fp64_oracle(S):
     """FP32 batch film against the FP64 oracle from the same initial height:
     rel-L2 of h <= 1e-5 after 400 steps; mass to 1e-6."""

@@ -415,3 +415,29 @@ def test_final_outputs_only_gives_the_same_state(S, T, variant, tau):
         assert np.array_equal(a[k], b[k]), k
     assert dha == dhb and ma == mb
     assert sa.keys() == sb.keys() and all(np.array_equal(sa[k], sb[k]) for k in sa)
+
+
+@pytest.mark.parametrize("variant", [0, 3], ids=["tiles", "strips"])
+def test_nontemporal_stores_give_the_same_bits(S, variant):
+    """nontemporal 0 / 1 / 2 change where stores are cached, never a value."""
+    import ctypes as C
+    from swalbe import _lib
+    Lx, Ly, n = 200, 130, 25
+    h0 = O.init_normal(Lx, Ly, 9, 1.0, 0.01)
+    got = []
+    for mode in (0, 1, 2):
+        sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=n, kbt=1e-6))
+        st = S.Sys(sysc, "GPU", T=np.float32)
+        st.height = h0
+        S.equilibrium(st, sysc)
+        o = _lib.StepOpts()
+        o.thermal, o.seed, o.variant, o.nontemporal = 1, 4, variant, mode
+        prm = sysc.param.cparams()
+        _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(prm), n, C.byref(o)))
+        got.append({k: st.get(k) for k in ("height", "velx", "vely", "pressure", "fout")})
+    o.nontemporal = 3
+    with pytest.raises(S.SwalbeError):
+        _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(prm), 1, C.byref(o)))
+    for g in got[1:]:
+        for k in g:
+            assert np.array_equal(g[k], got[0][k]), k
