@@ -487,8 +487,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the films are independent or a single slab; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (independent "
+              "films would shard by film; one large film by y-slabs with a halo exchange "
+              "per step)", file=sys.stderr)
         sys.exit(2)
 
     if a.workload == "coalesce256_batch":
