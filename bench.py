@@ -106,14 +106,23 @@ def cpu_baseline_fused(seconds_target: float = 8.0):
 def bench_batch(a):
     """256 independent droplet-coalescence films (Lx in {128..1024} step 64,
     Ly = Lx/2, two tangent caps, per-film θ, γ, radius) stepped together by
-    the batched FP32 kernel; K timed batch steps, run as captured graphs."""
+    the batched FP32 kernel; K timed batch steps, run as captured graphs.
+    Each film's tile height (8 or 16 rows) is the one the autotuner measured
+    fastest for its shape (the tuning runs at admission, before the timed
+    steps, and is reported)."""
     import swalbe as S
     from swalbe import serve
     reqs = serve.coalescence_sweep(256, seed=1, nsteps=a.warmup + a.steps)
     cap = sum(S.film_bytes(r.Lx, r.Ly) for r in reqs)
-    b = S.Batch(cap, 256, graph_steps=a.graph_steps)
-    for r in reqs:
-        b.add(r.param, r.Lx, r.Ly, r.init, r.nsteps)
+    b = S.Batch(cap, 256, graph_steps=a.graph_steps, autotune=not a.no_autotune)
+    t0 = time.perf_counter()
+    slots = [b.add(r.param, r.Lx, r.Ly, r.init, r.nsteps) for r in reqs]
+    b.sync()
+    admit_s = time.perf_counter() - t0
+    rows = [b.tile_y(sl) for sl in slots]
+    shapes = {}
+    for r, ty in zip(reqs, rows):
+        shapes[f"{r.Lx}x{r.Ly}"] = ty
     st = b.stats()
     b.step(a.warmup)
     b.sync()
@@ -142,6 +151,9 @@ def bench_batch(a):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_lup": bpl},
+        "autotune": {"on": not a.no_autotune, "shapes": len(shapes),
+                     "films_rows8": rows.count(8), "films_rows16": rows.count(16),
+                     "rows_by_shape": shapes, "admission_s": round(admit_s, 3)},
     }
     print(json.dumps(line), flush=True)
 
@@ -485,6 +497,8 @@ def main():
     ap.add_argument("--nontemporal", type=int, default=2,
                     help="FP32 stores: 0 cached, 1 all streaming, 2 f and p streaming")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="coalesce256_batch: every film in 16-row tiles")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (independent "

@@ -220,7 +220,8 @@ int swalbe_init_random(swalbe_state *st, int32_t dist, uint64_t seed, double bas
  * every swalbe_batch_step until the budget is spent and keeps its state
  * until removed.  Each film's arithmetic is the single-film FP32 kernel's,
  * so a film's result is the same alone, in any mix, and via swalbe_time_loop
- * with opts.variant = 2, opts.tile_y = 16.
+ * with opts.variant = 2 and opts.tile_y 8 or 16 (both heights give the same
+ * bits).
  * Initial height kinds follow src/initialvalues.jl (flat, randinterface!,
  * singledroplet, two_droplets); τ = 1 films keep one distribution buffer.
  * add / remove / step only enqueue work on the batch's stream (admissions
@@ -259,6 +260,25 @@ int swalbe_batch_step_timed(swalbe_batch *b, int64_t nsteps, double *ms);
 int swalbe_batch_set_graph(swalbe_batch *b, int32_t steps_per_graph);
 /* stores of the batch's films: 0, 1 or 2 as swalbe_step_opts.nontemporal */
 int swalbe_batch_set_nontemporal(swalbe_batch *b, int32_t on);
+/* Autotuned tile heights: every film runs in tiles of 64 columns by 8 or 16
+ * rows (16 by default).  With autotune on, a film admitted afterwards takes
+ * the height measured fastest for its shape: a temporary batch of copies of
+ * the film (about 4 M nodes) is timed with each height, once per device and
+ * (Lx, Ly, τ ≠ 1, noise, body force), and the result cached.  Timing a new
+ * shape synchronises the batch's stream first.  The heights give the same
+ * bits, so only the speed changes. */
+int swalbe_batch_set_autotune(swalbe_batch *b, int32_t on);
+/* films admitted afterwards use tiles of `rows` rows (8 or 16); 0 restores
+ * the default (16, or the tuned height with autotune on) */
+int swalbe_batch_set_tile_y(swalbe_batch *b, int32_t rows);
+/* tile height the tuner picks for this film shape on the current device
+ * (measured now, or cached), with the milliseconds per batched step of the
+ * copies at 8 and 16 rows (either pointer may be null) */
+int swalbe_batch_tune_shape(const swalbe_params *p, int32_t Lx, int32_t Ly,
+                            const swalbe_film_init *init, int32_t *tile_y, double *ms_rows8,
+                            double *ms_rows16);
+/* tile height (8 or 16) of a resident film */
+int swalbe_batch_film_tile_y(swalbe_batch *b, int32_t slot, int32_t *tile_y);
 /* Films admitted after this call with τ = 1 write their distributions and
  * pressure on their last step only (their heights and velocities every
  * step): the same final state for less traffic.  Downloading FOUT or

@@ -13,6 +13,8 @@
 #include <deque>
 #include <cmath>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "swalbe_batch.h"
@@ -113,6 +115,8 @@ struct swalbe_batch {
   int grid = 0;
   int nts = 0;
   int last_only = 0;  // films admitted with distributions / pressure written at their end only
+  int autotune = 0;   // films admitted with the tile height measured fastest for their shape
+  int forced_ty = 0;  // films admitted with this tile height (0: default or autotuned)
   int graph_steps = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -281,9 +285,123 @@ void release(swalbe_batch* b) {
   delete b;
 }
 
+// ---------------------------------------------------------------- autotuner
+// The tile height (8 or 16 rows) of a film shape is chosen by measurement:
+// a temporary batch of copies of the film (enough to fill the device, as in
+// a batch) is stepped with each height and the faster one kept.  Both
+// heights give the same bits, so the choice changes the speed only.
+// Results are cached per device and per (shape, τ ≠ 1, noise, body force).
+struct TuneKey {
+  int dev, Lx, Ly, read_ft, thermal, body;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dev, Lx, Ly, read_ft, thermal, body) <
+           std::tie(o.dev, o.Lx, o.Ly, o.read_ft, o.thermal, o.body);
+  }
+};
+struct TuneEntry {
+  int ty;
+  double ms8, ms16;  // per batched step of the copies
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tuned;
+
+constexpr double kTuneNodes = 4.0e6;  // nodes per timed batch of copies
+constexpr int kTuneWarm = 2, kTuneSteps = 8;
+
+// Milliseconds per batched step of `copies` copies of the film with tile
+// height ty: the faster of two timed runs of kTuneSteps steps.
+int time_copies(const swalbe_params* p, int Lx, int Ly, const swalbe_film_init* init, int ty,
+                int copies, double* ms_per_step) {
+  const size_t fb = film_bytes(Lx, Ly, p->tau != 1.0);
+  swalbe_batch* t = nullptr;
+  if (int rc = swalbe_batch_create(SWALBE_F32, fb * (size_t)copies + kAlign, copies, &t))
+    return rc;
+  t->forced_ty = ty;
+  int rc = SWALBE_OK;
+  for (int i = 0; i < copies && rc == SWALBE_OK; ++i) {
+    int32_t slot = 0;
+    rc = swalbe_batch_add(t, p, Lx, Ly, init, kTuneWarm + 2 * kTuneSteps, &slot);
+  }
+  double best = 0.0;
+  if (rc == SWALBE_OK) rc = swalbe_batch_step(t, kTuneWarm);
+  for (int r = 0; r < 2 && rc == SWALBE_OK; ++r) {
+    double ms = 0.0;
+    rc = swalbe_batch_step_timed(t, kTuneSteps, &ms);
+    if (rc == SWALBE_OK && (r == 0 || ms < best)) best = ms;
+  }
+  swalbe_batch_destroy(t);
+  if (rc == SWALBE_OK) *ms_per_step = best / kTuneSteps;
+  return rc;
+}
+
+int tuned_tile_y(const swalbe_params* p, int Lx, int Ly, const swalbe_film_init* init,
+                 TuneEntry* out) {
+  int dev = 0;
+  hipGetDevice(&dev);
+  const TuneKey key{dev, Lx, Ly, p->tau != 1.0 ? 1 : 0, init->thermal ? 1 : 0,
+                    init->body ? 1 : 0};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      *out = it->second;
+      return SWALBE_OK;
+    }
+  }
+  // as many copies as make ~4 M nodes, within a quarter of the free memory
+  const size_t fb = film_bytes(Lx, Ly, p->tau != 1.0);
+  int copies = (int)std::min(1024.0, std::max(1.0, std::ceil(kTuneNodes / ((double)Lx * Ly))));
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+    while (copies > 1 && fb * (size_t)copies > free_b / 4) copies /= 2;
+  TuneEntry e{kBatchTileY, 0.0, 0.0};
+  if (int rc = time_copies(p, Lx, Ly, init, 16, copies, &e.ms16)) return rc;
+  if (int rc = time_copies(p, Lx, Ly, init, 8, copies, &e.ms8)) return rc;
+  e.ty = e.ms8 < 0.98 * e.ms16 ? 8 : 16;  // 16 on a near tie (less halo work)
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = e;
+  }
+  *out = e;
+  return SWALBE_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int swalbe_batch_set_autotune(swalbe_batch* b, int32_t on) {
+  if (!b) return bad("null batch");
+  b->autotune = on ? 1 : 0;
+  return SWALBE_OK;
+}
+
+int swalbe_batch_set_tile_y(swalbe_batch* b, int32_t rows) {
+  if (!b || (rows != 0 && rows != 8 && rows != 16))
+    return bad("swalbe_batch_set_tile_y: rows is 0, 8 or 16");
+  b->forced_ty = rows;
+  return SWALBE_OK;
+}
+
+int swalbe_batch_tune_shape(const swalbe_params* p, int32_t Lx, int32_t Ly,
+                            const swalbe_film_init* init, int32_t* tile_y, double* ms_rows8,
+                            double* ms_rows16) {
+  if (!p || !init || !tile_y || Lx < 1 || Ly < 1 || p->n < 0 || p->m < 0)
+    return bad("swalbe_batch_tune_shape: bad arguments");
+  TuneEntry e{};
+  if (int rc = tuned_tile_y(p, Lx, Ly, init, &e)) return rc;
+  *tile_y = e.ty;
+  if (ms_rows8) *ms_rows8 = e.ms8;
+  if (ms_rows16) *ms_rows16 = e.ms16;
+  return SWALBE_OK;
+}
+
+int swalbe_batch_film_tile_y(swalbe_batch* b, int32_t slot, int32_t* tile_y) {
+  if (!b || !tile_y || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used)
+    return bad("swalbe_batch_film_tile_y: bad slot");
+  *tile_y = b->slots[slot].d.ty;
+  return SWALBE_OK;
+}
 
 int swalbe_batch_film_bytes(int32_t Lx, int32_t Ly, double tau, size_t* bytes) {
   if (!bytes || Lx < 1 || Ly < 1) return bad("swalbe_batch_film_bytes: bad arguments");
@@ -389,6 +507,24 @@ int swalbe_batch_add(swalbe_batch* b, const swalbe_params* p, int32_t Lx, int32_
     return bad("swalbe_batch_add: film too large");
   if (p->n < 0 || p->m < 0) return bad("swalbe_batch_add: negative exponent");
   hipSetDevice(b->device);
+  int ty = kBatchTileY;
+  if (b->forced_ty) {
+    ty = b->forced_ty;
+  } else if (b->autotune) {
+    TuneEntry e{};
+    bool cached = false;
+    {
+      int dev = 0;
+      hipGetDevice(&dev);
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      cached = g_tuned.count(TuneKey{dev, Lx, Ly, p->tau != 1.0 ? 1 : 0, init->thermal ? 1 : 0,
+                                     init->body ? 1 : 0}) > 0;
+    }
+    // a shape seen for the first time is timed on an idle device
+    if (!cached) SW_HIP(hipStreamSynchronize(b->stream));
+    if (int rc = tuned_tile_y(p, Lx, Ly, init, &e)) return rc;
+    ty = e.ty;
+  }
   int slot = -1;
   for (size_t i = 0; i < b->slots.size(); ++i)
     if (!b->slots[i].used) {
@@ -435,10 +571,11 @@ int swalbe_batch_add(swalbe_batch* b, const swalbe_params* p, int32_t Lx, int32_
     d.pairs = 0;
   } else {
     d.ntx = (Lx + 63) / 64;
-    d.ntiles = d.ntx * ((Ly + kBatchTileY - 1) / kBatchTileY);
-    d.small = film_is_small(Lx, Ly, 64, kBatchTileY);
+    d.ntiles = d.ntx * ((Ly + ty - 1) / ty);
+    d.small = film_is_small(Lx, Ly, 64, ty);
     d.pairs = (!d.small && (Ly % 2) == 0) ? 1 : 0;
   }
+  d.ty = ty;
   d.read_ft = two_f ? 1 : 0;
   d.last_only = b->last_only;
   d.thermal = init->thermal ? 1 : 0;

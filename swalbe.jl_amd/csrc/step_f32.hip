@@ -757,13 +757,16 @@ void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser
 // tiles (shared halos) stay in one L2.  Films are found by binary search of
 // the tile prefix table; each film's parity and step number follow from the
 // device launch counter, so the launch sequence can be a captured graph.
-template <int TY>
 __global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ films,
                                                   const int* __restrict__ tstart,
                                                   const BatchHdr* __restrict__ hdr,
                                                   const unsigned long long* __restrict__ counter,
                                                   int nts) {
-  __shared__ Smem<TX, TY, false> sm;
+  // LDS for the taller tile; a film of 8-row tiles uses its front part
+  static_assert(sizeof(Smem<TX, 8, false>) <= sizeof(Smem<TX, kBatchTileY, false>),
+                "8-row tiles fit the 16-row tile's LDS");
+  __shared__ Smem<TX, kBatchTileY, false> sm;
+  Smem<TX, 8, false>& sm8 = *reinterpret_cast<Smem<TX, 8, false>*>(&sm);
   const int total = hdr->total_tiles, nf = hdr->nfilms;
   const unsigned long long L = *counter;
   // each workgroup takes a contiguous run of its XCD's share: consecutive
@@ -827,14 +830,25 @@ __global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ 
     R.step = (uint64_t)F.t0 + n + 1;
     R.wet_thresh = 0.0;
     const int local = tile - tstart[lo];
-    const int x0 = (local % F.ntx) * TX, y0 = (local / F.ntx) * TY;
+    const int x0 = (local % F.ntx) * TX, y0 = (local / F.ntx) * F.ty;
     const F32K K = F.K;
-    if (F.read_ft) {
-      if (F.thermal) tile_step<TX, NT, TY, true, true, false, false>(io, K, R, x0, y0, sm, acc);
-      else tile_step<TX, NT, TY, true, false, false, false>(io, K, R, x0, y0, sm, acc);
+    if (F.ty == 8) {
+      if (F.read_ft) {
+        if (F.thermal) tile_step<TX, NT, 8, true, true, false, false>(io, K, R, x0, y0, sm8, acc);
+        else tile_step<TX, NT, 8, true, false, false, false>(io, K, R, x0, y0, sm8, acc);
+      } else {
+        if (F.thermal) tile_step<TX, NT, 8, false, true, false, false>(io, K, R, x0, y0, sm8, acc);
+        else tile_step<TX, NT, 8, false, false, false, false>(io, K, R, x0, y0, sm8, acc);
+      }
     } else {
-      if (F.thermal) tile_step<TX, NT, TY, false, true, false, false>(io, K, R, x0, y0, sm, acc);
-      else tile_step<TX, NT, TY, false, false, false, false>(io, K, R, x0, y0, sm, acc);
+      constexpr int TY = kBatchTileY;
+      if (F.read_ft) {
+        if (F.thermal) tile_step<TX, NT, TY, true, true, false, false>(io, K, R, x0, y0, sm, acc);
+        else tile_step<TX, NT, TY, true, false, false, false>(io, K, R, x0, y0, sm, acc);
+      } else {
+        if (F.thermal) tile_step<TX, NT, TY, false, true, false, false>(io, K, R, x0, y0, sm, acc);
+        else tile_step<TX, NT, TY, false, false, false, false>(io, K, R, x0, y0, sm, acc);
+      }
     }
     __syncthreads();
   }
@@ -1004,8 +1018,8 @@ void launch_batch_f32(const BatchFilm* films, const int* tstart, const BatchHdr*
     hipLaunchKernelGGL(k_batch_strip_f32, dim3(grid), dim3(64), 0, s, films, tstart, hdr, counter,
                        nts);
   else
-    hipLaunchKernelGGL(k_batch_f32<kBatchTileY>, dim3(grid), dim3(NT), 0, s, films, tstart, hdr,
-                       counter, nts);
+    hipLaunchKernelGGL(k_batch_f32, dim3(grid), dim3(NT), 0, s, films, tstart, hdr, counter,
+                       nts);
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, s, counter);
 }
 
@@ -1014,7 +1028,7 @@ int batch_blocks_per_cu() {
   const hipError_t e =
       kBatchStrips
           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_strip_f32, 64, 0)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_f32<kBatchTileY>, NT, 0);
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_f32, NT, 0);
   if (e != hipSuccess || per_cu < 1) per_cu = 1;
   return per_cu;
 }

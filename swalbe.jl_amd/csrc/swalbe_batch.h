@@ -16,7 +16,8 @@ struct F32K {
 F32K make_f32k(const StepParams& sp, const Enc& enc);
 int film_is_small(int Lx, int Ly, int tile_x, int tile_y);
 
-// Rows per tile of the batched kernel.
+// Rows per tile of the batched kernel: every film uses 8 or 16 (its own
+// choice, by default 16); the kernel's LDS holds the taller tile.
 constexpr int kBatchTileY = 16;
 // Default rows per segment of the strip kernel; films the strip kernel
 // treats with modulo wrapping (narrower than a strip or very short).
@@ -39,6 +40,7 @@ struct BatchFilm {
   // ntx, ntiles: strips and (strip, segment) items, or tile columns and tiles
   int Lx, Ly, ntx, ntiles, small, pairs, read_ft, thermal, body;
   int last_only;  // τ = 1: distributions and pressure written on the last step only
+  int ty;         // rows per tile of this film (8 or 16)
   int64_t fstride;
   uint64_t seed;
   int64_t t0;                // step number before admission

@@ -19,7 +19,7 @@ from .initialvalues import (Ellipsoid, VPolygon, boxpattern, ellipsepattern, nor
                             random_film, rivulet, sinewave, sinewave1D, singledroplet, torus,
                             trianglepattern, two_droplets_2d, uniforms)
 from .batch import (INIT_CAP, INIT_FLAT, INIT_NORMAL, INIT_TWO_CAPS, Batch, FilmInit, device_memory,
-                    film_bytes, film_init)
+                    film_bytes, film_init, tune_shape)
 from .io import load_state, restart_from_height, save_snapshots, save_state, snapshot
 from .simulate import (run_dropletforced, run_dropletpatterned, run_dropletrelax, run_flat,
                        run_gamma, run_random, run_rayleightaylor)

@@ -161,6 +161,11 @@ def lib():
     _sig(L, "swalbe_batch_set_graph", [_V, _I])
     _sig(L, "swalbe_batch_set_nontemporal", [_V, _I])
     _sig(L, "swalbe_batch_set_final_outputs_only", [_V, _I])
+    _sig(L, "swalbe_batch_set_autotune", [_V, _I])
+    _sig(L, "swalbe_batch_set_tile_y", [_V, _I])
+    _sig(L, "swalbe_batch_tune_shape", [C.POINTER(Params), _I, _I, C.POINTER(FilmInit),
+                                        C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)])
+    _sig(L, "swalbe_batch_film_tile_y", [_V, _I, C.POINTER(_I)])
     _sig(L, "swalbe_batch_sync", [_V])
     _sig(L, "swalbe_batch_wait", [_V, C.c_int64])
     _sig(L, "swalbe_batch_film_status", [_V, _I, C.POINTER(C.c_int64), C.POINTER(_I)])

@@ -44,6 +44,17 @@ def device_memory():
     return f.value, t.value
 
 
+def tune_shape(param, Lx: int, Ly: int, init) -> dict:
+    """The autotuner's choice for a film shape on the current device: tile
+    height (8 or 16 rows) and the milliseconds per batched step of ~4 M nodes
+    of copies at each height (measured once, then cached)."""
+    prm = param.cparams()
+    ty, m8, m16 = C.c_int32(), C.c_double(), C.c_double()
+    check(lib().swalbe_batch_tune_shape(C.byref(prm), int(Lx), int(Ly), C.byref(init),
+                                        C.byref(ty), C.byref(m8), C.byref(m16)), "tune_shape")
+    return {"tile_y": ty.value, "ms_rows8": m8.value, "ms_rows16": m16.value}
+
+
 def film_bytes(Lx: int, Ly: int, tau: float = 1.0) -> int:
     out = C.c_size_t()
     check(lib().swalbe_batch_film_bytes(int(Lx), int(Ly), float(tau), C.byref(out)), "film_bytes")
@@ -54,7 +65,7 @@ class Batch:
     """A device arena of `capacity_bytes` holding up to `max_films` films."""
 
     def __init__(self, capacity_bytes: int, max_films: int = 4096, graph_steps: int = 0,
-                 final_outputs_only: bool = False):
+                 final_outputs_only: bool = False, autotune: bool = False):
         h = C.c_void_p()
         check(lib().swalbe_batch_create(_lib.F32, int(capacity_bytes), int(max_films), C.byref(h)),
               "swalbe_batch_create")
@@ -62,6 +73,8 @@ class Batch:
         self.dims = {}
         if final_outputs_only:
             self.set_final_outputs_only(True)
+        if autotune:
+            self.set_autotune(True)
         if graph_steps:
             self.set_graph(graph_steps)
 
@@ -104,6 +117,21 @@ class Batch:
 
     def set_graph(self, steps: int) -> None:
         check(lib().swalbe_batch_set_graph(self._h, int(steps)), "set_graph")
+
+    def set_autotune(self, on: bool = True) -> None:
+        """Films added afterwards take the tile height (8 or 16 rows) measured
+        fastest for their shape; the heights give the same bits."""
+        check(lib().swalbe_batch_set_autotune(self._h, int(bool(on))), "set_autotune")
+
+    def set_tile_y(self, rows: int) -> None:
+        """Films added afterwards use tiles of `rows` rows (8 or 16); 0
+        restores the default (16, or the tuned height with autotune on)."""
+        check(lib().swalbe_batch_set_tile_y(self._h, int(rows)), "set_tile_y")
+
+    def tile_y(self, slot: int) -> int:
+        t = C.c_int32()
+        check(lib().swalbe_batch_film_tile_y(self._h, int(slot), C.byref(t)), "film_tile_y")
+        return t.value
 
     def set_nontemporal(self, mode: int) -> None:
         """Stores of the films (the same bits in every mode): 0 cached,

@@ -187,6 +187,45 @@ def test_graph_and_plain_launches_agree(S):
             assert np.array_equal(a, ref)
 
 
+def test_tile_heights_and_autotune_give_the_alone_bits(S):
+    """Films in 8-row tiles, 16-row tiles and tuned tiles (mixed in one
+    launch) are the same bits as each film alone; the tuner picks 8 or 16
+    for a shape, once, and reports both timings."""
+    specs = film_specs(S) + [(130, 20, S.Taumucs(Tmax=1, kbt=1e-6), S.film_init(
+        S.INIT_NORMAL, thermal=True, seed=16, base=1.0, amp=1e-3, href=1.0), 80)]
+    b = S.Batch(1 << 30, 64, graph_steps=16)
+    slots = []
+    for i, sp in enumerate(specs):
+        if i % 3 == 0:
+            b.set_tile_y(8)
+            b.set_autotune(False)
+        elif i % 3 == 1:
+            b.set_tile_y(16)
+        else:
+            b.set_tile_y(0)
+            b.set_autotune(True)
+        slots.append(b.add(*_args(sp)))
+    for i, (slot, sp) in enumerate(zip(slots, specs)):
+        ty = b.tile_y(slot)
+        assert ty in (8, 16)
+        if i % 3 == 0:
+            assert ty == 8
+        elif i % 3 == 1:
+            assert ty == 16
+        else:
+            t = S.tune_shape(sp[2], sp[0], sp[1], sp[3])
+            assert t["tile_y"] == ty and t["ms_rows8"] > 0 and t["ms_rows16"] > 0
+            assert S.tune_shape(sp[2], sp[0], sp[1], sp[3]) == t  # cached
+    b.step(max(s[4] for s in specs) + 3)
+    for slot, (Lx, Ly, prm, init, n) in zip(slots, specs):
+        print(f"[batch] tile rows {b.tile_y(slot)} film {Lx}x{Ly} steps {n}", flush=True)
+        got = {name: b.get(slot, name) for name in FIELDS}
+        assert_same(got, alone(S, Lx, Ly, prm, init, n), f"rows {b.tile_y(slot)} {Lx}x{Ly}")
+    with pytest.raises(S.SwalbeError):
+        b.set_tile_y(12)
+    b.close()
+
+
 def test_nontemporal_modes_give_the_same_bits(S):
     """The store mode is a cache hint only: every mode gives the same bits,
     also when it changes between runs of a captured graph (the graph is
