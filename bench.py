@@ -30,6 +30,7 @@ WORKLOADS = {
     "rupture4096_f32": (4096, 4096, "f32", True),
     "rupture4096_f64": (4096, 4096, "f64", True),
     "flat4096_f32_nonoise": (4096, 4096, "f32", False),
+    "flat4096_f64_nonoise": (4096, 4096, "f64", False),
     "droplet1024_f32": (1024, 1024, "f32", False),
 }
 # many-film workloads (FP32 batched kernel), the 1D coalescence scan and the copy calibration
@@ -588,8 +589,9 @@ def main():
         "vs_baseline": None,
         "dtype": dt,
         "data": "synthetic (seeded Philox film h0 = 1 + 0.001 N(0,1))",
-        "config": {"workload": f"{a.workload}: D2Q9 thin-film rupture with thermal noise, "
-                               f"{Lx}x{Ly} periodic, kbt=1e-6, theta=1/9, n=9, m=3, gamma=0.01, tau=1",
+        "config": {"workload": f"{a.workload}: D2Q9 thin film "
+                               + ("rupture with thermal noise, kbt=1e-6, " if thermal else "without noise, ")
+                               + f"{Lx}x{Ly} periodic, theta=1/9, n=9, m=3, gamma=0.01, tau=1",
                    "Lx": Lx, "Ly": Ly, "parallelism": "single GPU", "tile_y": a.tile_y or 16,
                    "variant": a.variant, "nontemporal": a.nontemporal,
                    "plane_pad": int(os.environ.get("SWALBE_PLANE_PAD", "0")), "finite": ok},
