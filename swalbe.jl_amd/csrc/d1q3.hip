@@ -10,6 +10,7 @@
 //     each) in one launch.
 // FP64 evaluates every expression in the reference's order (the library is
 // built with -ffp-contract=off); FP32 stores plain values.
+#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -722,16 +723,19 @@ void split_dists(swalbe_film1d* s, int field) {
 }
 
 int64_t max_lds_bytes(int device) {
-  static int64_t cache[64] = {0};
+  // per-device cache; threads that race here store the same value
+  static std::atomic<int64_t> cache[64];
   if (device < 0 || device >= 64) return 65536;
-  if (!cache[device]) {
+  int64_t c = cache[device].load(std::memory_order_relaxed);
+  if (!c) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess ||
         v <= 0)
       v = 65536;
-    cache[device] = v;
+    c = v;
+    cache[device].store(c, std::memory_order_relaxed);
   }
-  return cache[device];
+  return c;
 }
 
 int64_t resident_bytes(int L, size_t esz, bool read_ft) {
