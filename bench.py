@@ -571,6 +571,16 @@ def main():
     o.final_outputs_only = 0
     h = st.get("height", np.float64)
     ok = bool(np.isfinite(h).all())
+    # the same number of steps with the film handed over by the host and the
+    # height handed back (PCIe both ways, on the wall clock); never `value`
+    st.synchronize()
+    w0 = time.perf_counter()
+    st.height = h
+    o.t0 = a.warmup + 2 * a.steps
+    _lib.check(L.swalbe_time_loop(st.handle, C.byref(prm), a.steps, C.byref(o)))
+    h_back = st.get("height", np.float64)
+    pcie_s = time.perf_counter() - w0
+    ok = ok and bool(np.isfinite(h_back).all())
     ceil = ceilings(st, a.tile_y or 16) if (Lx, Ly, dt) == (4096, 4096, "f32") else None
     if ceil:
         ceil["frac_of_copy"] = round(achieved / ceil["copy_GBs"], 4)
@@ -602,6 +612,10 @@ def main():
                                "bytes_per_lup": 7 * esz,
                                "note": "f and p written on the last step only (opts)"},
         "ceilings": ceil,
+        "host_handover": {"GLUPS": round(N * a.steps / pcie_s / 1e9, 3),
+                          "seconds": round(pcie_s, 4),
+                          "note": "height uploaded, the same steps, height downloaded (PCIe "
+                                  "both ways, wall clock)"},
     }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
