@@ -154,8 +154,9 @@ typedef struct swalbe_step_opts {
     double *dh_series;     /* host [nsteps]: max(h) - min(h) before each step (:56), or NULL */
     int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) (measures.jl:13-17), or NULL */
     double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
-    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16); 8 or 16; variant 3:
-                              rows per strip segment, 0 = default (64), 4..4096 */
+    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16 for FP32, 8 for FP64);
+                              8 or 16 (the same results either way); variant 3: rows per
+                              strip segment, 0 = default (64), 4..4096 */
     int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = LDS tiles, collide at the
                               source node, 2 = LDS tiles, collide at the destination node,
                               3 = LDS-free strips (one wavefront per 58-column strip) */

@@ -441,3 +441,32 @@ def test_nontemporal_stores_give_the_same_bits(S, variant):
     for g in got[1:]:
         for k in g:
             assert np.array_equal(g[k], got[0][k]), k
+
+
+@pytest.mark.parametrize("tile", [8, 16])
+def test_fp64_tile_heights_bitexact(S, tile):
+    """The FP64 step is the oracle's arithmetic at either tile height (8 is
+    the FP64 default): noise, θ field and τ ≠ 1 on a ragged film."""
+    import ctypes as C
+    from swalbe import _lib
+    Lx, Ly, n = 130, 70, 60
+    print(f"[parity] fp64 tile {tile}", flush=True)
+    h0 = O.init_normal(Lx, Ly, 5, 1.0, 0.05)
+    theta = np.asfortranarray(np.random.default_rng(3).uniform(1 / 18, 1 / 6, (Lx, Ly)))
+    prm = O.params(tau=0.9, gamma=0.01, kbt=1e-4, tdump=1000)
+    ost = _oracle_state(h0)
+    ost.ftemp[...] = ost.feq
+    O.time_loop(ost, prm, n, theta_field=theta, thermal=True, seed=21)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=n, τ=0.9, γ=0.01, kbt=1e-4))
+    st = S.Sys(sysc, "GPU")
+    st.height = h0
+    S.equilibrium(st, sysc)
+    st.copy("ftemp", "feq")
+    st.theta = theta
+    o = _lib.StepOpts()
+    o.thermal, o.seed, o.tile_y, o.use_theta_field = 1, 21, tile, 1
+    prmc = sysc.param.cparams()
+    _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(prmc), n, C.byref(o)))
+    for name, ref in (("height", ost.height), ("velx", ost.velx), ("pressure", ost.pressure),
+                      ("fout", ost.fout)):
+        assert np.array_equal(st.get(name), ref), name
