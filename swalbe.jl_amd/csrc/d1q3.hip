@@ -390,6 +390,14 @@ __global__ __launch_bounds__(NB) void k1_step(Film1 S, uint64_t step,
   ((T*)S.p_out)[x] = sp[t + 2];
 }
 
+// Pointers read from the film table are generic to the compiler, which then
+// emits flat loads and stores (they wait on the LDS counter as well).  All of
+// them address global memory: say so.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* glb(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 // Whole films in LDS: workgroup b steps film table[b] through nsteps steps.
 // LDS: h, u, h', u', p, F (and f_t, f_t' when τ ≠ 1), L words each (3L for f).
 // Only what every step uses is held in registers; the series pointers,
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(kRT) __attribute__((amdgpu_waves_per_eu(kRT == 512 
 void k1_resident(const Film1* table, int64_t nsteps) {
   extern __shared__ __align__(16) unsigned char smem1[];
   __shared__ double red[4][kRT / 64];
-  const Film1* F1 = table + blockIdx.x;
+  const auto* F1 = glb(table + blockIdx.x);
   const int L = F1->L, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int read_ft = F1->read_ft;
   T* h = (T*)smem1;
@@ -413,14 +421,14 @@ void k1_resident(const Film1* table, int64_t nsteps) {
   T* ft = Fo + L;
   T* ftn = ft + 3 * L;
   {
-    const T* h_in = (const T*)F1->h_in;
-    const T* u_in = (const T*)F1->u_in;
+    const auto* h_in = glb((const T*)F1->h_in);
+    const auto* u_in = glb((const T*)F1->u_in);
     for (int x = tid; x < L; x += kRT) {
       h[x] = h_in[x];
       u[x] = u_in[x];
     }
     if (read_ft) {
-      const T* ft_in = (const T*)F1->ft;
+      const auto* ft_in = glb((const T*)F1->ft);
       for (int i = tid; i < 3 * L; i += kRT) ft[i] = ft_in[i];
     }
   }
@@ -429,11 +437,11 @@ void k1_resident(const Film1* table, int64_t nsteps) {
   const T gam_s = (T)F1->gamma, kap_s = (T)F1->kap_s, hmin = (T)F1->hmin, hcrit = (T)F1->hcrit;
   const T c6mu = (T)F1->c6mu, c6d = (T)F1->c6d, c3dd = (T)F1->c3dd;
   const T g = (T)F1->g, omeg = (T)F1->omeg, itau = (T)F1->itau;
-  const T* kap_f = (const T*)F1->kap;
-  const T* gam_f = (const T*)F1->gam;
-  const T* gg = (const T*)F1->gg;
-  const T* b1 = (const T*)F1->b1;
-  const T* b2 = (const T*)F1->b2;
+  const auto* kap_f = glb((const T*)F1->kap);
+  const auto* gam_f = glb((const T*)F1->gam);
+  const auto* gg = glb((const T*)F1->gg);
+  const auto* b1 = glb((const T*)F1->b1);
+  const auto* b2 = glb((const T*)F1->b2);
   const bool series = F1->dh || F1->wet || F1->mass;
   // countdowns to the next mass (t % tdump == 0) and snapshot step
   int64_t cm = F1->mass ? F1->mass_phase : -1, cs = F1->snap ? F1->snap_phase : -1;
@@ -445,7 +453,7 @@ void k1_resident(const Film1* table, int64_t nsteps) {
     const bool last = it == nsteps;
     const bool do_mass = cm == 0;
     if (series) {  // series over the step's input height
-      const T* inter = (const T*)F1->interior;
+      const auto* inter = glb((const T*)F1->interior);
       const double wthr = F1->wet_thresh;
       double s = 0.0, mx = -__builtin_inf(), mn = __builtin_inf(), c = 0.0;
       for (int x = tid; x < L; x += kRT) {
@@ -474,9 +482,9 @@ void k1_resident(const Film1* table, int64_t nsteps) {
           n0 = red[2][w] < n0 ? red[2][w] : n0;
           c0 += red[3][w];
         }
-        if (do_mass) F1->mass[km] = s0;
-        if (F1->dh) F1->dh[it - 1] = m0 - n0;
-        if (F1->wet) F1->wet[it - 1] = c0;
+        if (do_mass) glb(F1->mass)[km] = s0;
+        if (F1->dh) glb(F1->dh)[it - 1] = m0 - n0;
+        if (F1->wet) glb(F1->wet)[it - 1] = c0;
       }
     }
     if (do_mass) {
@@ -531,15 +539,15 @@ void k1_resident(const Film1* table, int64_t nsteps) {
         ftn[2 * L + x] = f2;
       }
       if (last) {
-        T* fout = (T*)F1->f_out;
+        auto* fout = glb((T*)F1->f_out);
         fout[x] = f0;
         fout[L + x] = f1;
         fout[2 * L + x] = f2;
-        ((T*)F1->h_out)[x] = hv;
-        ((T*)F1->u_out)[x] = uv;
-        ((T*)F1->p_out)[x] = p[x];
-        ((T*)F1->h_last)[x] = h[x];
-        ((T*)F1->u_last)[x] = u[x];
+        glb((T*)F1->h_out)[x] = hv;
+        glb((T*)F1->u_out)[x] = uv;
+        glb((T*)F1->p_out)[x] = p[x];
+        glb((T*)F1->h_last)[x] = h[x];
+        glb((T*)F1->u_last)[x] = u[x];
       }
     }
     __syncthreads();
@@ -553,7 +561,7 @@ void k1_resident(const Film1* table, int64_t nsteps) {
     ft = ftn;
     ftn = tmp;
     if (cs == 0) {  // snapshot!(…, h, t) after the step
-      double* snap = F1->snap + ks * L;
+      auto* snap = glb(F1->snap + ks * L);
       for (int x = tid; x < L; x += kRT) snap[x] = (double)h[x];
       ++ks;
       cs = F1->snap_every;
