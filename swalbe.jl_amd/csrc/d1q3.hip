@@ -399,7 +399,9 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* glb(T* p) {
 }
 
 // Whole films in LDS: workgroup b steps film table[b] through nsteps steps.
-// LDS: h, u, h', u', p, F (and f_t, f_t' when τ ≠ 1), L words each (3L for f).
+// LDS: h, u, h', u', p, F (and f_t, f_t' when τ ≠ 1), L words each (3L for f);
+// within a step the collision at the source overwrites h, u and F with the
+// three post-collision populations, which the stream phase gathers.
 // Only what every step uses is held in registers; the series pointers,
 // last-step outputs and noise constants are read from the table where they
 // are used (the table is not restrict: the loop's global stores keep those
@@ -498,36 +500,40 @@ void k1_resident(const Film1* table, int64_t nsteps) {
                            kap_f ? kap_f[x] : kap_s, n, m, hmin, hcrit, 0);
     }
     __syncthreads();
+    // forces, then the collision at the source: node x's post-collision
+    // populations replace its own h, u (no other node reads them in this
+    // phase) and F: o0 -> h[x], o1 -> u[x], o2 -> Fo[x]
     for (int x = tid; x < L; x += kRT) {
       const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
-      const T hh = h[x];
+      const T hh = h[x], uu = u[x];
       const T hgp = (hh * T(-0.5)) * (p[xm] - p[xp]);
-      T F = -hgp - slip_node(hh, u[x], c6mu, c6d, c3dd);
+      T F = -hgp - slip_node(hh, uu, c6mu, c6d, c3dd);
       if (thermal)
         F = F - noise_node<T>(F1->seed, (uint64_t)(F1->t0 + it), x) *
                     noise_amp(hh, (T)F1->kb26, (T)F1->delta);
       if (gg) F = gg_sign < 0 ? F - gg[x] : F + gg[x];
       if (body) F = F + (hh * (T)F1->alpha) * (T)F1->fac;
-      Fo[x] = F;
+      if (last) {  // the last step's input moments
+        glb((T*)F1->h_last)[x] = hh;
+        glb((T*)F1->u_last)[x] = uu;
+      }
+      T e0, e1, e2;
+      feq_node(hh, uu, g, e0, e1, e2);
+      const T t0 = read_ft ? ft[x] : T(0), t1 = read_ft ? ft[L + x] : T(0),
+              t2 = read_ft ? ft[2 * L + x] : T(0);
+      h[x] = omeg * t0 + itau * e0;
+      u[x] = (omeg * t1 + itau * e1) + T(0.5) * F;
+      Fo[x] = (omeg * t2 + itau * e2) - T(0.5) * F;
     }
     __syncthreads();
+    // pull-stream (bounce-back at walls) and moments
     for (int x = tid; x < L; x += kRT) {
       const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
-      auto fo = [&](int k, int y) -> T {
-        T e0, e1, e2;
-        feq_node(h[y], u[y], g, e0, e1, e2);
-        const T fe = k == 0 ? e0 : (k == 1 ? e1 : e2);
-        const T tk = read_ft ? ft[k * L + y] : T(0);
-        T v = omeg * tk + itau * fe;
-        if (k == 1) v = v + T(0.5) * Fo[y];
-        if (k == 2) v = v - T(0.5) * Fo[y];
-        return v;
-      };
-      const T f0 = fo(0, x);
-      T f1 = fo(1, xm), f2 = fo(2, xp);
+      const T f0 = h[x];
+      T f1 = u[xm], f2 = Fo[xp];
       if (b1) {
-        f1 = (f1 - f1 * b1[xm]) + fo(2, x) * b2[x];
-        f2 = (f2 - f2 * b2[xp]) + fo(1, x) * b1[x];
+        f1 = (f1 - f1 * b1[xm]) + Fo[x] * b2[x];
+        f2 = (f2 - f2 * b2[xp]) + u[x] * b1[x];
       }
       const T hv = (f0 + f1) + f2;
       const T uv = (f1 - f2) / hv;
@@ -546,8 +552,6 @@ void k1_resident(const Film1* table, int64_t nsteps) {
         glb((T*)F1->h_out)[x] = hv;
         glb((T*)F1->u_out)[x] = uv;
         glb((T*)F1->p_out)[x] = p[x];
-        glb((T*)F1->h_last)[x] = h[x];
-        glb((T*)F1->u_last)[x] = u[x];
       }
     }
     __syncthreads();
