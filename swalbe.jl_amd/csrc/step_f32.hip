@@ -392,7 +392,9 @@ __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const
                                           int x0, int y0, Smem<TXW, TY, SRC>& sm,
                                           SeriesAcc& acc) {
   Halo<TXW, NTH, TY> hr;
+  __builtin_amdgcn_s_setprio(2);  // issue the tile's loads ahead of other waves' arithmetic
   halo_load<TXW, NTH, TY>(io, R, x0, y0, hr);
+  __builtin_amdgcn_s_setprio(0);
   halo_commit<TXW, NTH, TY, SRC>(hr, sm);
   __syncthreads();
   tile_compute<TXW, NTH, TY, READ_FT, THERMAL, SRC, SERIES>(io, K, R, x0, y0, sm, acc);
