@@ -322,6 +322,7 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
   }
   __syncthreads();
 
+  __builtin_amdgcn_s_setprio(1);  // the stores ahead of other waves' pressure and forces
   // ---- D: stream (gather) and moments; a wave takes whole 64-column rows
   for (int rc = wv; rc < TY * (TXW / 64); rc += NW) {
     const int row = rc / (TXW / 64), col = lane + 64 * (rc - row * (TXW / 64));
@@ -381,6 +382,7 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
       acc.cnt += hv > R.wet_thresh ? 1u : 0u;
     }
   }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 
