@@ -317,8 +317,10 @@ __global__ __launch_bounds__(NB) void k1_step(Film1 S, uint64_t step,
   // a workgroup whose 3-halo wraps nowhere indexes directly (no modulo)
   const bool inner = x0 >= 3 && x0 + NB + 3 <= L;
   auto wr = [&](int y) { return inner ? y : wrapL(y, L); };
+  __builtin_amdgcn_s_setprio(2);  // the loads ahead of other waves' arithmetic
   for (int i = t; i < NB + 6; i += NB) sh[i] = h_in[wr(x0 - 3 + i)];
   for (int i = t; i < NB + 2; i += NB) su[i] = u_in[wr(x0 - 1 + i)];
+  __builtin_amdgcn_s_setprio(0);
   __syncthreads();
   for (int i = t; i < NB + 4; i += NB) {  // p at node x0 - 2 + i
     const int y = wr(x0 - 2 + i);

@@ -134,6 +134,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
     constexpr int HN = HR * HC, UN = UR * UC;
     constexpr int NHI = (HN + NT - 1) / NT, NUI = (UN + NT - 1) / NT;
     T hreg[NHI], ureg[NUI], vreg[NUI];
+    __builtin_amdgcn_s_setprio(2);  // issue the tile's loads ahead of other waves' arithmetic
     if (!small && x0 >= 3 && y0 >= 3 && x0 + TX + 3 <= Lx && y0 + TY + 3 <= Ly &&
         Lx <= (1 << 23)) {
       const char* hb = (const char*)(io.h_in + ((x0 - 3) + (int64_t)Lx * (y0 - 3)));
@@ -183,6 +184,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
         }
       }
     }
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int i = 0; i < NHI; ++i) {
       const int e = tid + i * NT;
@@ -292,6 +294,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   }
   __syncthreads();
 
+  __builtin_amdgcn_s_setprio(1);  // the stores ahead of other waves' pressure and forces
   // ---- phase D: collide the source populations, pull-stream, moments
   const int tx = tid & (TX - 1);
   const int ty = tid >> 6;
@@ -345,6 +348,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
     }
   }
 
+  __builtin_amdgcn_s_setprio(0);
   // ---- per-step series: max/min/count of h_t over the domain (exact)
   if (ser.dh_keys || ser.wet) {
     unsigned long long kmax = dkey(lmax), kmin = dkey(lmin);
