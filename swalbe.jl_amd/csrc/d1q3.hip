@@ -409,7 +409,7 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* glb(T* p) {
 // are used (the table is not restrict: the loop's global stores keep those
 // loads in place), and the mass / snapshot schedule runs on countdowns.
 template <typename T, int kRT>
-__global__ __launch_bounds__(kRT) __attribute__((amdgpu_waves_per_eu(kRT == 512 ? 5 : 1)))
+__global__ __launch_bounds__(kRT) __attribute__((amdgpu_waves_per_eu(kRT == 512 ? 4 : 1)))
 void k1_resident(const Film1* table, int64_t nsteps) {
   extern __shared__ __align__(16) unsigned char smem1[];
   __shared__ double red[4][kRT / 64];
@@ -447,6 +447,35 @@ void k1_resident(const Film1* table, int64_t nsteps) {
   const auto* b1 = glb((const T*)F1->b1);
   const auto* b2 = glb((const T*)F1->b2);
   const bool series = F1->dh || F1->wet || F1->mass;
+  // The per-node constants every step reads (γ, ∇γ and the wall masks
+  // around x) are held in registers for each thread's first kC nodes;
+  // further nodes, and a θ field, are read from global memory each step.
+  constexpr int kC = kRT <= 512 ? 2 : 0;
+  constexpr int kCs = kC > 0 ? kC : 1;
+  T c_gam[kCs], c_gg[kCs], c_b1m[kCs], c_b1[kCs], c_b2[kCs], c_b2p[kCs];
+#pragma unroll
+  for (int j = 0; j < kC; ++j) {
+    const int x = tid + j * kRT;
+    if (x < L) {
+      const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
+      c_gam[j] = gam_f ? gam_f[x] : gam_s;
+      c_gg[j] = gg ? gg[x] : T(0);
+      c_b1m[j] = b1 ? b1[xm] : T(0);
+      c_b1[j] = b1 ? b1[x] : T(0);
+      c_b2[j] = b2 ? b2[x] : T(0);
+      c_b2p[j] = b2 ? b2[xp] : T(0);
+    }
+  }
+  // body(x, j) for every node of this thread; j is the register slot of a
+  // cached node (a constant after unrolling) or -1
+  auto for_nodes = [&](auto&& body) {
+#pragma unroll
+    for (int j = 0; j < kC; ++j) {
+      const int x = tid + j * kRT;
+      if (x < L) body(x, j);
+    }
+    for (int x = tid + kC * kRT; x < L; x += kRT) body(x, -1);
+  };
   // countdowns to the next mass (t % tdump == 0) and snapshot step
   int64_t cm = F1->mass ? F1->mass_phase : -1, cs = F1->snap ? F1->snap_phase : -1;
   int64_t km = 0, ks = 0;
@@ -496,16 +525,17 @@ void k1_resident(const Film1* table, int64_t nsteps) {
       cm = F1->tdump;
     }
     if (cm > 0) --cm;
-    for (int x = tid; x < L; x += kRT) {
+    for_nodes([&](int x, int j) {
       const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
-      p[x] = pressure_node(h[xm], h[x], h[xp], gam_f ? gam_f[x] : gam_s,
-                           kap_f ? kap_f[x] : kap_s, n, m, hmin, hcrit, 0);
-    }
+      const T gm = j >= 0 ? c_gam[j] : (gam_f ? gam_f[x] : gam_s);
+      const T kp = kap_f ? kap_f[x] : kap_s;
+      p[x] = pressure_node(h[xm], h[x], h[xp], gm, kp, n, m, hmin, hcrit, 0);
+    });
     __syncthreads();
     // forces, then the collision at the source: node x's post-collision
     // populations replace its own h, u (no other node reads them in this
     // phase) and F: o0 -> h[x], o1 -> u[x], o2 -> Fo[x]
-    for (int x = tid; x < L; x += kRT) {
+    for_nodes([&](int x, int j) {
       const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
       const T hh = h[x], uu = u[x];
       const T hgp = (hh * T(-0.5)) * (p[xm] - p[xp]);
@@ -513,7 +543,10 @@ void k1_resident(const Film1* table, int64_t nsteps) {
       if (thermal)
         F = F - noise_node<T>(F1->seed, (uint64_t)(F1->t0 + it), x) *
                     noise_amp(hh, (T)F1->kb26, (T)F1->delta);
-      if (gg) F = gg_sign < 0 ? F - gg[x] : F + gg[x];
+      if (gg) {
+        const T gv = j >= 0 ? c_gg[j] : gg[x];
+        F = gg_sign < 0 ? F - gv : F + gv;
+      }
       if (body) F = F + (hh * (T)F1->alpha) * (T)F1->fac;
       if (last) {  // the last step's input moments
         glb((T*)F1->h_last)[x] = hh;
@@ -526,16 +559,18 @@ void k1_resident(const Film1* table, int64_t nsteps) {
       h[x] = omeg * t0 + itau * e0;
       u[x] = (omeg * t1 + itau * e1) + T(0.5) * F;
       Fo[x] = (omeg * t2 + itau * e2) - T(0.5) * F;
-    }
+    });
     __syncthreads();
     // pull-stream (bounce-back at walls) and moments
-    for (int x = tid; x < L; x += kRT) {
+    for_nodes([&](int x, int j) {
       const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
       const T f0 = h[x];
       T f1 = u[xm], f2 = Fo[xp];
       if (b1) {
-        f1 = (f1 - f1 * b1[xm]) + Fo[x] * b2[x];
-        f2 = (f2 - f2 * b2[xp]) + u[x] * b1[x];
+        const T bm = j >= 0 ? c_b1m[j] : b1[xm], bx2 = j >= 0 ? c_b2[j] : b2[x];
+        const T bp = j >= 0 ? c_b2p[j] : b2[xp], bx1 = j >= 0 ? c_b1[j] : b1[x];
+        f1 = (f1 - f1 * bm) + Fo[x] * bx2;
+        f2 = (f2 - f2 * bp) + u[x] * bx1;
       }
       const T hv = (f0 + f1) + f2;
       const T uv = (f1 - f2) / hv;
@@ -555,7 +590,7 @@ void k1_resident(const Film1* table, int64_t nsteps) {
         glb((T*)F1->u_out)[x] = uv;
         glb((T*)F1->p_out)[x] = p[x];
       }
-    }
+    });
     __syncthreads();
     T* tmp = h;
     h = hn;
@@ -1223,8 +1258,9 @@ hipError_t launch_resident_t(const Film1* d_table, int nfilms, int64_t nsteps, i
 }
 
 // threads per film (0 = automatic).  Measured on MI355X, FP64, L = 1024:
-// with at least one film per CU, 512 threads (5 waves per SIMD fit by
-// registers, 2 films per CU) step the ensemble fastest; a few films are
+// with at least one film per CU, 512 threads (2 films per CU: 8 waves each,
+// up to 128 VGPRs for the per-node constants held in registers) step the
+// ensemble fastest (155 against 137 GLUPS at 256 threads); a few films are
 // latency bound and run fastest at one node per thread (1024).
 int resident_threads(int requested, int maxL, int nfilms, int device) {
   if (requested == 256 || requested == 512 || requested == 1024) return requested;
