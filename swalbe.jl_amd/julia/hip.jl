@@ -417,3 +417,209 @@ function time_loop(sys::SysConstWithBound_1D, s::HipState1D; verbose = false, di
     hip_walls!(s, sys)
     return fused1d!(sys, s, HipStepOpts1D(t0 = s.t, bound = 1))
 end
+
+# ---- raw-array methods on caller-owned device arrays --------------------------
+# The reference's positional methods (src/equilibrium.jl:63, src/collide.jl:70,
+# src/moments.jl:43, src/pressure.jl:72, src/differences.jl:57-206,
+# src/forcing.jl:42, :297), for AMDGPU.jl ROCArrays in Julia's layout.  Each
+# method is one swalbe_*_ptr call, enqueued on AMDGPU's task-local stream, so it
+# orders with the caller's own broadcasts on the same arrays.
+using AMDGPU: ROCArray, ROCMatrix
+hipptr(A) = A === nothing ? C_NULL : reinterpret(Ptr{Cvoid}, pointer(A))
+hipstream() = AMDGPU.stream().stream
+hipdt(::Type{Float64}) = HIP_F64
+hipdt(::Type{Float32}) = HIP_F32
+
+function equilibrium!(feq::ROCArray{T,3}, height::ROCMatrix{T}, velx::ROCMatrix{T},
+                      vely::ROCMatrix{T}, vsq::ROCMatrix{T}, gravity) where {T}
+    Lx, Ly = size(height)
+    hip_check(ccall((:swalbe_equilibrium_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64,
+                 Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(feq), hipptr(height), hipptr(velx), hipptr(vely), hipptr(vsq),
+                gravity, Lx, Ly, hipstream()), "equilibrium!")
+    return nothing
+end
+function BGKandStream!(fout::ROCArray{T,3}, feq::ROCArray{T,3}, ftemp::ROCArray{T,3},
+                       Fx::ROCMatrix{T}, Fy::ROCMatrix{T}, τ) where {T}
+    Lx, Ly = size(Fx)
+    hip_check(ccall((:swalbe_bgk_and_stream_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64,
+                 Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(fout), hipptr(feq), hipptr(ftemp), hipptr(Fx), hipptr(Fy), τ,
+                Lx, Ly, hipstream()), "BGKandStream!")
+    return nothing
+end
+function moments!(height::ROCMatrix{T}, velx::ROCMatrix{T}, vely::ROCMatrix{T},
+                  fout::ROCArray{T,3}) where {T}
+    Lx, Ly = size(height)
+    hip_check(ccall((:swalbe_moments_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(height), hipptr(velx), hipptr(vely), hipptr(fout), Lx, Ly,
+                hipstream()), "moments!")
+    return nothing
+end
+# the raw form: (n, m) ∈ {(9, 3), (3, 2)} only, DomainError otherwise (src/pressure.jl:102-107)
+function filmpressure!(output::ROCMatrix{T}, f::ROCMatrix{T}, dgrad, γ, θ, n, m, hmin,
+                       hcrit) where {T}
+    Lx, Ly = size(f)
+    field = θ isa ROCMatrix ? hipptr(θ) : C_NULL
+    θs = θ isa Number ? Float64(θ) : 0.0
+    hip_check(ccall((:swalbe_filmpressure_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Float64, Ptr{Cvoid}, Int32, Int32,
+                 Float64, Float64, Int32, Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(output), hipptr(f), γ, θs, field, n, m, hmin, hcrit, 1, Lx, Ly,
+                hipstream()), "filmpressure!")
+    return nothing
+end
+function hip_grad!(outputx::ROCMatrix{T}, outputy::ROCMatrix{T}, f::ROCMatrix{T}, a) where {T}
+    Lx, Ly = size(f)
+    hip_check(ccall((:swalbe_grad_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(outputx), hipptr(outputy), hipptr(f), hipptr(a), Lx, Ly,
+                hipstream()), "∇f!")
+    return nothing
+end
+∇f!(outputx::ROCMatrix, outputy::ROCMatrix, f::ROCMatrix) = hip_grad!(outputx, outputy, f, nothing)
+∇f!(outputx::ROCMatrix, outputy::ROCMatrix, f::ROCMatrix, a::ROCMatrix) =
+    hip_grad!(outputx, outputy, f, a)
+∇f!(outputx::ROCMatrix, outputy::ROCMatrix, f::ROCMatrix, dgrad, a::ROCMatrix) =
+    hip_grad!(outputx, outputy, f, a)
+# a scalar `a` scales the unweighted stencil: one product either way, the same
+# bits in FP64 (in FP32 the stencil is rounded before the product)
+function ∇f!(outputx::ROCMatrix, outputy::ROCMatrix, f::ROCMatrix, a::Number)
+    hip_grad!(outputx, outputy, f, nothing)
+    outputx .*= a
+    outputy .*= a
+    return nothing
+end
+∇f!(outputx::ROCMatrix, outputy::ROCMatrix, f::ROCMatrix, dgrad, a::Number) =
+    ∇f!(outputx, outputy, f, a)
+function ∇²f!(output::ROCMatrix{T}, f::ROCMatrix{T}, γ) where {T}
+    Lx, Ly = size(f)
+    hip_check(ccall((:swalbe_laplacian_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(output), hipptr(f), γ, Lx, Ly, hipstream()), "∇²f!")
+    return nothing
+end
+function slippage!(slipx::ROCMatrix{T}, slipy::ROCMatrix{T}, height::ROCMatrix{T},
+                   velx::ROCMatrix{T}, vely::ROCMatrix{T}, δ, μ) where {T}
+    Lx, Ly = size(height)
+    hip_check(ccall((:swalbe_slippage_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64,
+                 Float64, Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(slipx), hipptr(slipy), hipptr(height), hipptr(velx),
+                hipptr(vely), δ, μ, Lx, Ly, hipstream()), "slippage!")
+    return nothing
+end
+# the normals come from the counter-based stream (seed, step, node), not Julia's randn!
+function thermal!(fluc_x::ROCMatrix{T}, fluc_y::ROCMatrix{T}, height::ROCMatrix{T}, kᵦT, μ, δ;
+                  seed = 0, step = 0) where {T}
+    Lx, Ly = size(height)
+    hip_check(ccall((:swalbe_thermal_ptr, libswalbe), Cint,
+                (Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Float64, Float64, UInt64,
+                 UInt64, Int32, Int32, Ptr{Cvoid}),
+                hipdt(T), hipptr(fluc_x), hipptr(fluc_y), hipptr(height), kᵦT, μ, δ, seed, step,
+                Lx, Ly, hipstream()), "thermal!")
+    return nothing
+end
+synchronize(s::HipState) =
+    hip_check(ccall((:swalbe_synchronize, libswalbe), Cint, (Ptr{Cvoid},), s.handle), "synchronize")
+
+# ---- batches and the server (no reference counterpart: the reference runs a
+# parameter sweep as a serial loop of run_* calls, scripts/Rivulet_stability.jl:141-190)
+# swalbe_film_init
+struct HipFilmInit
+    kind::Int32; thermal::Int32; body::Int32; clamp::Int32
+    seed::UInt64
+    base::Float64; amp::Float64; precursor::Float64; href::Float64
+    r1::Float64; theta1::Float64; c1x::Float64; c1y::Float64
+    r2::Float64; theta2::Float64; c2x::Float64; c2y::Float64
+    body_ax::Float64; body_ay::Float64; body_fac::Float64
+end
+# kind: 0 flat, 1 base + amp N(0,1), 2 one cap, 3 two caps (1-based centres, θ in units of π)
+HipFilmInit(; kind = 0, thermal = 0, body = 0, clamp = 0, seed = 0, base = 1.0, amp = 0.0,
+         precursor = 0.05, href = 0.0, r1 = 0.0, theta1 = 0.0, c1x = 0.0, c1y = 0.0, r2 = 0.0,
+         theta2 = 0.0, c2x = 0.0, c2y = 0.0, body_ax = 0.0, body_ay = 0.0, body_fac = 0.0) =
+    HipFilmInit(kind, thermal, body, clamp, seed, base, amp, precursor, href, r1, theta1, c1x,
+             c1y, r2, theta2, c2x, c2y, body_ax, body_ay, body_fac)
+
+# FP32 films of any sizes in one arena, stepped together by one launch per step
+mutable struct HipBatch
+    handle::Ptr{Cvoid}
+    function HipBatch(capacity_bytes; max_films = 4096, graph_steps = 0, autotune = true)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        hip_check(ccall((:swalbe_batch_create, libswalbe), Cint,
+                    (Int32, Csize_t, Int32, Ref{Ptr{Cvoid}}), HIP_F32, capacity_bytes, max_films,
+                    out), "batch create")
+        b = finalizer(new(out[])) do x
+            x.handle == C_NULL || ccall((:swalbe_batch_destroy, libswalbe), Cint, (Ptr{Cvoid},),
+                                        x.handle)
+        end
+        autotune && hip_check(ccall((:swalbe_batch_set_autotune, libswalbe), Cint,
+                                    (Ptr{Cvoid}, Int32), b.handle, 1), "autotune")
+        graph_steps > 0 && hip_check(ccall((:swalbe_batch_set_graph, libswalbe), Cint,
+                                           (Ptr{Cvoid}, Int32), b.handle, graph_steps), "graph")
+        return b
+    end
+end
+function Base.push!(b::HipBatch, sys::SysConst, init::HipFilmInit, nsteps)
+    slot = Ref{Int32}(0)
+    hip_check(ccall((:swalbe_batch_add, libswalbe), Cint,
+                (Ptr{Cvoid}, Ref{HipParams}, Int32, Int32, Ref{HipFilmInit}, Int64, Ref{Int32}),
+                b.handle, HipParams(sys.param), sys.Lx, sys.Ly, init, nsteps, slot), "batch add")
+    return slot[]
+end
+Base.delete!(b::HipBatch, slot) =
+    hip_check(ccall((:swalbe_batch_remove, libswalbe), Cint, (Ptr{Cvoid}, Int32), b.handle, slot),
+          "batch remove")
+step!(b::HipBatch, nsteps) =
+    hip_check(ccall((:swalbe_batch_step, libswalbe), Cint, (Ptr{Cvoid}, Int64), b.handle, nsteps),
+          "batch step")
+function batch_height(b::HipBatch, slot, Lx, Ly)
+    h = Matrix{Float64}(undef, Lx, Ly)
+    hip_check(ccall((:swalbe_batch_download, libswalbe), Cint,
+                (Ptr{Cvoid}, Int32, Int32, Ptr{Cvoid}, Csize_t), b.handle, slot,
+                HIP_FIELDS[:height], h, sizeof(h)), "batch download")
+    return h
+end
+
+# continuous batching of solve requests: submit, then poll until each is done
+mutable struct HipServer
+    handle::Ptr{Cvoid}
+    function HipServer(capacity_bytes; max_films = 4096, tick = 500, graph_steps = 0)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        hip_check(ccall((:swalbe_serve_create, libswalbe), Cint,
+                    (Csize_t, Int32, Int32, Int32, Ref{Ptr{Cvoid}}), capacity_bytes, max_films,
+                    tick, graph_steps, out), "serve create")
+        return finalizer(new(out[])) do x
+            x.handle == C_NULL || ccall((:swalbe_serve_destroy, libswalbe), Cint, (Ptr{Cvoid},),
+                                        x.handle)
+        end
+    end
+end
+function submit!(s::HipServer, sys::SysConst, init::HipFilmInit, nsteps)
+    id = Ref{Int64}(0)
+    hip_check(ccall((:swalbe_serve_submit, libswalbe), Cint,
+                (Ptr{Cvoid}, Ref{HipParams}, Int32, Int32, Ref{HipFilmInit}, Int64, Ref{Int64}),
+                s.handle, HipParams(sys.param), sys.Lx, sys.Ly, init, nsteps, id), "submit")
+    return id[]
+end
+# one tick: admit, enqueue, hand back what finished; returns [(id, height), ...]
+function poll!(s::HipServer)
+    done, resident, queued = Ref{Int32}(0), Ref{Int32}(0), Ref{Int32}(0)
+    hip_check(ccall((:swalbe_serve_poll, libswalbe), Cint,
+                (Ptr{Cvoid}, Ref{Int32}, Ref{Int32}, Ref{Int32}), s.handle, done, resident, queued),
+          "poll")
+    out = Tuple{Int64,Matrix{Float64}}[]
+    for _ in 1:done[]
+        id, Lx, Ly = Ref{Int64}(0), Ref{Int32}(0), Ref{Int32}(0)
+        hip_check(ccall((:swalbe_serve_peek, libswalbe), Cint,
+                    (Ptr{Cvoid}, Ref{Int64}, Ref{Int32}, Ref{Int32}), s.handle, id, Lx, Ly), "peek")
+        h = Matrix{Float64}(undef, Lx[], Ly[])
+        hip_check(ccall((:swalbe_serve_take, libswalbe), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
+                    s.handle, h, sizeof(h)), "take")
+        push!(out, (id[], h))
+    end
+    return out
+end

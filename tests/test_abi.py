@@ -102,3 +102,33 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_julia_structs_match_the_ctypes_mirrors():
+    # The Julia binding's isbits structs are passed by Ref to the same entry
+    # points: read their fields from hip.jl and lay them out as C does
+    # (natural alignment); each must give the ctypes mirror's offsets and size.
+    import ctypes as C
+    import re
+
+    from swalbe import _lib
+    src = open(os.path.join(ROOT, "swalbe.jl_amd", "julia", "hip.jl"), encoding="utf-8").read()
+    width = {"Int32": 4, "UInt32": 4, "Int64": 8, "UInt64": 8, "Float64": 8, "Float32": 4,
+             "Ptr{Float64}": 8, "Ptr{Int64}": 8, "Ptr{UInt8}": 8, "Ptr{Cvoid}": 8}
+    pairs = {"HipParams": _lib.Params, "HipStepOpts": _lib.StepOpts,
+             "HipStepOpts1D": _lib.StepOpts1D, "HipFilmInit": _lib.FilmInit}
+    for jname, py in pairs.items():
+        m = re.search(r"^struct %s\n(.*?)^end" % jname, src, re.S | re.M)
+        assert m, jname
+        body = re.sub(r"#.*", "", m.group(1))
+        fields = re.findall(r"([^\s;:]+)::([A-Za-z0-9{}]+)", body)
+        off, layout = 0, []
+        for _, t in fields:
+            w = width[t]
+            off = (off + w - 1) // w * w
+            layout.append(off)
+            off += w
+        size = (off + 7) // 8 * 8
+        assert len(fields) == len(py._fields_), jname
+        assert layout == [getattr(py, f).offset for f, _ in py._fields_], jname
+        assert size == C.sizeof(py), jname
