@@ -476,6 +476,25 @@ void k1_resident(const Film1* table, int64_t nsteps) {
     }
     for (int x = tid + kC * kRT; x < L; x += kRT) body(x, -1);
   };
+  auto pressure_phase = [&]() {
+    for_nodes([&](int x, int j) {
+      const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
+      const T gm = j >= 0 ? c_gam[j] : (gam_f ? gam_f[x] : gam_s);
+      const T kp = kap_f ? kap_f[x] : kap_s;
+      p[x] = pressure_node(h[xm], h[x], h[xp], gm, kp, n, m, hmin, hcrit, 0);
+    });
+  };
+  // A film without walls computes the next step's pressure in the stream
+  // phase: each thread also streams the heights of its nodes' two
+  // neighbours (the same sums, so the same bits), which saves the barrier
+  // between the moments and the pressure.  Walled films keep three phases,
+  // and so do 1024-thread films: one node per thread, run when a few films
+  // hold the GPU, where the longer stream phase costs more than the barrier.
+  const bool fuse_p = kRT <= 512 && b1 == nullptr;
+  if (fuse_p) {
+    pressure_phase();
+    __syncthreads();
+  }
   // countdowns to the next mass (t % tdump == 0) and snapshot step
   int64_t cm = F1->mass ? F1->mass_phase : -1, cs = F1->snap ? F1->snap_phase : -1;
   int64_t km = 0, ks = 0;
@@ -525,13 +544,10 @@ void k1_resident(const Film1* table, int64_t nsteps) {
       cm = F1->tdump;
     }
     if (cm > 0) --cm;
-    for_nodes([&](int x, int j) {
-      const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
-      const T gm = j >= 0 ? c_gam[j] : (gam_f ? gam_f[x] : gam_s);
-      const T kp = kap_f ? kap_f[x] : kap_s;
-      p[x] = pressure_node(h[xm], h[x], h[xp], gm, kp, n, m, hmin, hcrit, 0);
-    });
-    __syncthreads();
+    if (!fuse_p) {
+      pressure_phase();
+      __syncthreads();
+    }
     // forces, then the collision at the source: node x's post-collision
     // populations replace its own h, u (no other node reads them in this
     // phase) and F: o0 -> h[x], o1 -> u[x], o2 -> Fo[x]
@@ -589,6 +605,13 @@ void k1_resident(const Film1* table, int64_t nsteps) {
         glb((T*)F1->h_out)[x] = hv;
         glb((T*)F1->u_out)[x] = uv;
         glb((T*)F1->p_out)[x] = p[x];
+      } else if (fuse_p) {  // the next step's pressure (no walls here)
+        const int xmm = xm == 0 ? L - 1 : xm - 1, xpp = xp == L - 1 ? 0 : xp + 1;
+        const T hvm = (h[xm] + u[xmm]) + Fo[x];
+        const T hvp = (h[xp] + u[x]) + Fo[xpp];
+        const T gm = j >= 0 ? c_gam[j] : (gam_f ? gam_f[x] : gam_s);
+        const T kp = kap_f ? kap_f[x] : kap_s;
+        p[x] = pressure_node(hvm, hv, hvp, gm, kp, n, m, hmin, hcrit, 0);
       }
     });
     __syncthreads();

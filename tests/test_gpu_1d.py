@@ -258,6 +258,29 @@ def test_per_step_workgroup_sizes_bitwise_vs_oracle(S, L, name, threads):
         assert np.array_equal(np.array(dh), out["dh"])
 
 
+@pytest.mark.parametrize("threads", [256, 512])
+@pytest.mark.parametrize("L,name", [(1030, "plain"), (1300, "thermal"), (1500, "walls"),
+                                    (1100, "gamma_plus"), (1200, "theta_field")],
+                         ids=lambda v: str(v))
+def test_resident_nodes_per_thread_bitwise_vs_oracle(S, L, name, threads):
+    """The resident kernel with several nodes per thread: the first two held
+    with their per-node constants in registers, the rest read each step;
+    films with and without walls (the latter compute the next step's
+    pressure in the stream phase).  The oracle's bits, Δh series included."""
+    cfg = CONFIGS[name]
+    n = 120
+    print(f"1D resident {name} L={L} threads {threads}", flush=True)
+    sysc, prm, st, ost, okw, hkw, args = _setup(S, L, n, cfg, variant=2)
+    hkw["threads"] = threads
+    out = O.time_loop_1d(ost, prm, n, **okw)
+    dh = _run_hip(S, sysc, st, hkw, args, cfg.get("body"))
+    for f_h, f_o in (("height", "height"), ("vel", "vel"), ("fout", "fout"),
+                     ("pressure", "pressure")):
+        assert np.array_equal(st.get(f_h), getattr(ost, f_o)), (name, f_h)
+    if isinstance(dh, list) and dh:
+        assert np.array_equal(np.array(dh), out["dh"])
+
+
 @pytest.mark.parametrize("L", [3, 255, 257, 1000])
 def test_fp64_lengths_and_both_loops_agree(S, L):
     print(f"1D lengths L={L}", flush=True)
@@ -349,6 +372,31 @@ def test_ensemble_equals_single_films_and_oracle(S):
                          γ=[f[1].get("γ") for f in films],
                          grad_gamma_sign=[f[1].get("grad_gamma_sign", 0) for f in films],
                          θ=[f[2][0] if f[2] else None for f in films])
+    for (st, _, _), ost, (L, name) in zip(films, oracles, specs):
+        assert np.array_equal(st.height, ost.height), name
+        assert np.array_equal(st.fout, ost.fout), name
+        assert np.array_equal(st.pressure, ost.pressure), name
+
+
+def test_ensemble_two_nodes_per_thread_equals_oracle(S):
+    # 512 threads per film and films longer than 512 nodes: every thread
+    # holds two nodes (the scan benchmark's shape), walled and open films mixed
+    specs = [(1100, "plain"), (700, "thermal"), (1024, "gamma_plus"), (900, "walls"),
+             (600, "theta_field")]
+    n = 100
+    films, systems, oracles = [], [], []
+    for L, name in specs:
+        sysc, prm, st, ost, okw, hkw, args = _setup(S, L, n, CONFIGS[name], variant=2)
+        films.append((st, hkw, args))
+        systems.append(sysc)
+        O.time_loop_1d(ost, prm, n, **okw)
+        oracles.append(ost)
+    print(f"1D ensemble of {len(films)} films, 512 threads each", flush=True)
+    S.ensemble_time_loop(systems, [f[0] for f in films], n,
+                         thermal=[f[1]["thermal"] for f in films], seed=11,
+                         γ=[f[1].get("γ") for f in films],
+                         grad_gamma_sign=[f[1].get("grad_gamma_sign", 0) for f in films],
+                         θ=[f[2][0] if f[2] else None for f in films], threads=512)
     for (st, _, _), ost, (L, name) in zip(films, oracles, specs):
         assert np.array_equal(st.height, ost.height), name
         assert np.array_equal(st.fout, ost.fout), name
