@@ -362,7 +362,7 @@ def ceilings(st, tile_y):
     """What the part allows for this step, measured live beside it: a plain
     device-to-device copy of nine 4096^2 FP32 planes (GB/s, read + write) and
     the step's own access pattern with no arithmetic (read 3 / write 13 planes
-    over the same 64 x tile_y tiles, GLUPS)."""
+    over the same 64 x tile_y tiles, every store streaming, GLUPS)."""
     import ctypes as C
     from swalbe import _lib
     for _ in range(3):
@@ -378,13 +378,15 @@ def ceilings(st, tile_y):
     _lib.check(_lib.lib().swalbe_pattern_probe(4096, 4096, 3, 13, tile_y, 50, C.byref(ms)), "probe")
     return {"copy_GBs": round(copy_gbs, 1),
             "pattern_GLUPS": round(4096 * 4096 / (ms.value / 1e3) / 1e9, 3),
-            "pattern": f"read 3 / write 13 planes over 64x{tile_y} tiles, no arithmetic"}
+            "pattern": f"read 3 / write 13 planes over 64x{tile_y} tiles, no arithmetic, "
+                       "streaming stores"}
 
 
 def bench_pattern(a):
     """The fused step's data movement alone at 4096^2 FP32: read 3 planes (h,
     ux, uy), write 13 (9 f, h, ux, uy, p) over the step's 64 x 16 tiles, no
-    arithmetic: the ceiling the step's access pattern allows."""
+    arithmetic, every store streaming: the ceiling the step's access pattern
+    allows."""
     import ctypes as C
     from swalbe import _lib
     ms = C.c_double()
@@ -399,7 +401,8 @@ def bench_pattern(a):
                       "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                       "dtype": "f32", "data": "synthetic",
                       "config": {"workload": "pattern4096_f32: read 3 / write 13 planes of "
-                                             f"4096^2 FP32 over 64x{ty} tiles, no arithmetic"},
+                                             f"4096^2 FP32 over 64x{ty} tiles, no arithmetic, "
+                                             "streaming stores"},
                       "achieved_GBs": round(gbs, 1)}), flush=True)
 
 
