@@ -10,12 +10,15 @@
 #     state = Swalbe.Sys(sys, "HIP"; T = Float64)        # a HipState on the GPU
 #     Swalbe.equilibrium!(state, sys)
 #     Swalbe.time_loop(sys, state)                       # one fused kernel per step
-#     h = state.height                                   # downloads on access
+#     h = state.height                                   # a live view (HipField)
 #
 # Every function below is a more specific method of a Swalbe function for
 # HipState; the call sites in src/simulate.jl (time_loop, run_*) and user
-# scripts stay as they are.  `state.height` downloads a copy; assignments
-# `state.height = A` or `state.height = 1.0` upload or fill on the device.
+# scripts stay as they are.  `state.height` is a HipField, an AbstractArray
+# view of the device array: reads see the device's values and in-place writes
+# (`state.height .= 1.0`, `state.height[i, j] = v`, randinterface! and
+# singledroplet writing into it, `state.ftemp .= state.fout`) reach the device
+# before the next call on the state.
 
 # path of libswalbe_hip.so (swalbe.jl_amd/lib/ after its build)
 const libswalbe = get(ENV, "SWALBE_HIP_LIB", "libswalbe_hip")
@@ -44,7 +47,7 @@ struct HipStepOpts
     seed::UInt64; t0::Int64
     body_ax::Float64; body_ay::Float64; body_fac::Float64; wet_thresh::Float64
     dh_series::Ptr{Float64}; wet_series::Ptr{Int64}; mass_series::Ptr{Float64}
-    tile_y::Int32; variant::Int32; nontemporal::Int32; pad_::Int32
+    tile_y::Int32; variant::Int32; nontemporal::Int32; no_fp64_guard::Int32
     snap_every::Int64; snap_series::Ptr{Float64}
     maxh_series::Ptr{Float64}; area_series::Ptr{Float64}; energy_series::Ptr{Float64}
     fluid_series::Ptr{UInt8}; surf_thresh::Float64; surf_theta_field::Int32
@@ -68,14 +71,122 @@ function hip_check(rc::Cint, what)
     error("$what: $msg (status $rc)")
 end
 
-"""GPU-resident State{T,3} / State_thermal{T,3}: the library owns the arrays."""
+# ---- state fields that read and write through ---------------------------------
+# The reference's drivers and scripts write into the state's arrays in place
+# (src/simulate.jl:241, 289, 350-353, 394; scripts/Rivulet_stability.jl:94-95).
+# A state keeps a Float64 host mirror of each field read or written element by
+# element: downloaded on first use, edited in place, marked dirty.  Every call
+# into the library goes through `state.handle`, which uploads the dirty
+# mirrors and marks all mirrors stale (the call may change any field), so the
+# next read downloads again.  Whole-field writes (fill!, copyto!, `.=`) go to
+# the device directly.  FP32 states decode to and re-encode from Float64
+# exactly, so untouched nodes of an uploaded mirror keep their bits.
+mutable struct HipMirror
+    A::Array{Float64}
+    version::Int
+    dirty::Bool
+end
+
+struct HipField{T,N,S} <: AbstractArray{T,N}
+    s::S
+    name::Symbol
+    dims::NTuple{N,Int}
+end
+Base.size(f::HipField) = f.dims
+Base.IndexStyle(::Type{<:HipField}) = IndexLinear()
+
+function hip_mirror(f::HipField)
+    s = f.s
+    ms = getfield(s, :mirrors)
+    v = getfield(s, :version)
+    m = get(ms, f.name, nothing)
+    if m === nothing || (m.version != v && !m.dirty)
+        A = Array{Float64}(undef, f.dims)
+        hip_download!(s, f.name, A)
+        m = HipMirror(A, v, false)
+        ms[f.name] = m
+    end
+    return m
+end
+
+Base.getindex(f::HipField{T}, i::Int) where {T} = T(hip_mirror(f).A[i])
+function Base.setindex!(f::HipField, x, i::Int)
+    m = hip_mirror(f)
+    m.A[i] = x
+    m.dirty = true
+    return f
+end
+
+# upload the fields written on the host, mark every mirror stale and return
+# the library handle: the path of every call that may read or change the state
+function hip_flush!(s)
+    h = getfield(s, :handle)
+    for (name, m) in getfield(s, :mirrors)
+        if m.dirty
+            hip_upload!(s, name, m.A)
+            m.dirty = false
+        end
+    end
+    setfield!(s, :version, getfield(s, :version) + 1)
+    return h
+end
+
+# whole-field writes go straight to the device
+function hip_set!(f::HipField, A::AbstractArray)
+    length(A) == prod(f.dims) || throw(DimensionMismatch("$(size(A)) into $(f.dims)"))
+    B = Array{Float64}(undef, f.dims)
+    copyto!(B, A)  # read first: A may be a field of the same state
+    hip_flush!(f.s)
+    delete!(getfield(f.s, :mirrors), f.name)
+    hip_upload!(f.s, f.name, B)
+    return f
+end
+function Base.fill!(f::HipField, x::Number)
+    hip_flush!(f.s)
+    delete!(getfield(f.s, :mirrors), f.name)
+    hip_fill!(f.s, f.name, Float64(x))
+    return f
+end
+Base.copyto!(dest::HipField, src::AbstractArray) = hip_set!(dest, src)
+# `state.X .= expr`: a scalar fills on the device, anything else is evaluated
+# on the host (reading fields through their mirrors) and uploaded whole
+function Base.copyto!(dest::HipField, bc::Base.Broadcast.Broadcasted{Nothing})
+    axes(dest) == axes(bc) || throw(DimensionMismatch("broadcast into state.$(dest.name)"))
+    if bc.f === identity && length(bc.args) == 1 && bc.args[1] isa Number
+        return fill!(dest, bc.args[1])
+    end
+    B = Array{Float64}(undef, dest.dims)
+    copyto!(B, bc)
+    return hip_set!(dest, B)
+end
+Base.Array(f::HipField{T}) where {T} = T.(hip_mirror(f).A)
+Base.copy(f::HipField) = Array(f)
+# whole-array reductions on one host copy instead of one mirror lookup per element
+Base.sum(f::HipField; kw...) = sum(Array(f); kw...)
+Base.maximum(f::HipField; kw...) = maximum(Array(f); kw...)
+Base.minimum(f::HipField; kw...) = minimum(Array(f); kw...)
+
+"""GPU-resident State{T,3} / State_thermal{T,3}: the library owns the arrays;
+`state.<field>` is a HipField."""
 mutable struct HipState{T} <: LBM_state_2D
     handle::Ptr{Cvoid}
     Lx::Int
     Ly::Int
     t::Int
     thermal::Bool
+    version::Int
+    mirrors::Dict{Symbol,HipMirror}
 end
+
+hip_download!(s::HipState, name::Symbol, A::Array{Float64}) =
+    hip_check(ccall((:swalbe_download, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
+                getfield(s, :handle), HIP_FIELDS[name], A, sizeof(A)), "download $name")
+hip_upload!(s::HipState, name::Symbol, A::Array{Float64}) =
+    hip_check(ccall((:swalbe_upload, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
+                getfield(s, :handle), HIP_FIELDS[name], A, sizeof(A)), "upload $name")
+hip_fill!(s::HipState, name::Symbol, x::Float64) =
+    hip_check(ccall((:swalbe_fill, libswalbe), Cint, (Ptr{Cvoid}, Int32, Float64),
+                getfield(s, :handle), HIP_FIELDS[name], x), "fill $name")
 
 function HipState{T}(Lx, Ly; kind = "simple", href = 1.0, g = 0.0) where {T}
     out = Ref{Ptr{Cvoid}}(C_NULL)
@@ -83,36 +194,32 @@ function HipState{T}(Lx, Ly; kind = "simple", href = 1.0, g = 0.0) where {T}
                 (Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
                 Lx, Ly, T === Float32 ? HIP_F32 : HIP_F64, kind == "thermal" ? HIP_THERMAL : HIP_SIMPLE, out),
           "swalbe_create")
-    s = HipState{T}(out[], Lx, Ly, 0, kind == "thermal")
+    s = HipState{T}(out[], Lx, Ly, 0, kind == "thermal", 0, Dict{Symbol,HipMirror}())
     T === Float32 && hip_check(ccall((:swalbe_set_reference_height, libswalbe), Cint,
                                  (Ptr{Cvoid}, Float64, Float64), s.handle, href, g),
                            "set_reference_height")
     finalizer(s) do x
-        x.handle == C_NULL || ccall((:swalbe_destroy, libswalbe), Cint, (Ptr{Cvoid},), x.handle)
-        x.handle = C_NULL
+        h = getfield(x, :handle)
+        h == C_NULL || ccall((:swalbe_destroy, libswalbe), Cint, (Ptr{Cvoid},), h)
+        setfield!(x, :handle, C_NULL)
     end
     return s
 end
 
 function Base.getproperty(s::HipState{T}, name::Symbol) where {T}
+    name === :handle && return hip_flush!(s)
     name in fieldnames(HipState) && return getfield(s, name)
     haskey(HIP_FIELDS, name) || error("HipState has no field $name")
-    dims = name in HIP_DISTS ? (s.Lx, s.Ly, 9) : (s.Lx, s.Ly)
-    A = Array{T}(undef, dims)
-    hip_check(ccall((:swalbe_download, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
-                getfield(s, :handle), HIP_FIELDS[name], A, sizeof(A)), "download $name")
-    return A
+    dims = name in HIP_DISTS ? (getfield(s, :Lx), getfield(s, :Ly), 9) :
+           (getfield(s, :Lx), getfield(s, :Ly))
+    return HipField{T,length(dims),HipState{T}}(s, name, dims)
 end
 
 function Base.setproperty!(s::HipState{T}, name::Symbol, A) where {T}
     name in fieldnames(HipState) && return setfield!(s, name, A)
-    if A isa Number
-        return hip_check(ccall((:swalbe_fill, libswalbe), Cint, (Ptr{Cvoid}, Int32, Float64),
-                           s.handle, HIP_FIELDS[name], A), "fill $name")
-    end
-    B = convert(Array{T}, A)
-    hip_check(ccall((:swalbe_upload, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
-                s.handle, HIP_FIELDS[name], B, sizeof(B)), "upload $name")
+    f = getproperty(s, name)
+    A isa HipField && A.s === s && A.name === name && return A  # after an in-place op
+    return A isa Number ? fill!(f, A) : hip_set!(f, A)
 end
 
 # ---- ops (src/equilibrium.jl:118, src/collide.jl:151, src/moments.jl:64,
@@ -288,40 +395,50 @@ HipStepOpts1D(; use_theta_field = 0, use_gamma_field = 0, grad_gamma = 0, therma
                mass_series, snap_every, snap_series, 0, 0)
 
 """GPU-resident State_1D (and its thermal / gamma kinds): the library owns the
-arrays; `s.basestate` is `s` itself."""
+arrays; `s.basestate` is `s` itself and `s.<field>` is a HipField."""
 mutable struct HipState1D{T} <: LBM_state_1D
     handle::Ptr{Cvoid}
     L::Int
     t::Int
+    version::Int
+    mirrors::Dict{Symbol,HipMirror}
 end
 function HipState1D{T}(L; kind = "simple") where {T}
     h = Ref{Ptr{Cvoid}}(C_NULL)
     hip_check(ccall((:swalbe1d_create, libswalbe), Cint, (Int32, Int32, Int32, Ref{Ptr{Cvoid}}),
                 L, T === Float32 ? HIP_F32 : HIP_F64,
                 kind == "thermal" ? HIP_THERMAL : HIP_SIMPLE, h), "Sys (1D)")
-    x = HipState1D{T}(h[], L, 0)
+    x = HipState1D{T}(h[], L, 0, 0, Dict{Symbol,HipMirror}())
     finalizer(x) do x
-        x.handle == C_NULL || ccall((:swalbe1d_destroy, libswalbe), Cint, (Ptr{Cvoid},), x.handle)
+        h = getfield(x, :handle)
+        h == C_NULL || ccall((:swalbe1d_destroy, libswalbe), Cint, (Ptr{Cvoid},), h)
+        setfield!(x, :handle, C_NULL)
     end
     return x
 end
-function Base.getproperty(s::HipState1D{T}, name::Symbol) where {T}
-    name in (:handle, :L, :t) && return getfield(s, name)
-    name === :basestate && return s
-    A = name in (:fout, :ftemp, :feq) ? zeros(Float64, s.L, 3) : zeros(Float64, s.L)
+hip_download!(s::HipState1D, name::Symbol, A::Array{Float64}) =
     hip_check(ccall((:swalbe1d_download, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
-                s.handle, HIP1D_FIELDS[name], A, sizeof(A)), "download $name")
-    return T.(A)
+                getfield(s, :handle), HIP1D_FIELDS[name], A, sizeof(A)), "download $name")
+hip_upload!(s::HipState1D, name::Symbol, A::Array{Float64}) =
+    hip_check(ccall((:swalbe1d_upload, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
+                getfield(s, :handle), HIP1D_FIELDS[name], A, sizeof(A)), "upload $name")
+hip_fill!(s::HipState1D, name::Symbol, x::Float64) =
+    hip_check(ccall((:swalbe1d_fill, libswalbe), Cint, (Ptr{Cvoid}, Int32, Float64),
+                getfield(s, :handle), HIP1D_FIELDS[name], x), "fill $name")
+function Base.getproperty(s::HipState1D{T}, name::Symbol) where {T}
+    name === :handle && return hip_flush!(s)
+    name in fieldnames(HipState1D) && return getfield(s, name)
+    name === :basestate && return s
+    haskey(HIP1D_FIELDS, name) || error("HipState1D has no field $name")
+    L = getfield(s, :L)
+    dims = name in (:fout, :ftemp, :feq) ? (L, 3) : (L,)
+    return HipField{T,length(dims),HipState1D{T}}(s, name, dims)
 end
 function Base.setproperty!(s::HipState1D, name::Symbol, A)
-    name in (:handle, :L, :t) && return setfield!(s, name, A)
-    if A isa Number
-        return hip_check(ccall((:swalbe1d_fill, libswalbe), Cint, (Ptr{Cvoid}, Int32, Float64),
-                           s.handle, HIP1D_FIELDS[name], A), "fill $name")
-    end
-    B = Float64.(A)
-    hip_check(ccall((:swalbe1d_upload, libswalbe), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t),
-                s.handle, HIP1D_FIELDS[name], B, sizeof(B)), "upload $name")
+    name in fieldnames(HipState1D) && return setfield!(s, name, A)
+    f = getproperty(s, name)
+    A isa HipField && A.s === s && A.name === name && return A  # after an in-place op
+    return A isa Number ? fill!(f, A) : hip_set!(f, A)
 end
 
 function hip_walls!(s::HipState1D, sys::SysConstWithBound_1D)

@@ -16,6 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import DIST_FIELDS, FIELDS, check, lib
+from .fields import HipField, LiveFields
 
 def _one_d():
     from . import one_d
@@ -80,12 +81,16 @@ class SysConst:
 # ---------------------------------------------------------------- state
 
 
-class HipState:
+class HipState(LiveFields):
     """A film resident on the GPU: every array of State{T,3}
     (src/initialize.jl:149-168) plus kbtx/kbty and an optional θ field.
 
-    Reading `state.height` downloads a copy; assigning an array uploads it and
-    assigning a scalar fills the field (state.height .= 1.0)."""
+    `state.height` is a live view of the device array (a HipField): reads
+    see the device's values and in-place writes (`state.height[...] = 1.0`,
+    `state.height[i, j] = v`, `state.ftemp[...] = state.fout`,
+    `state.Fx += a`) reach the device before the next op, as the reference's
+    in-place writes into its state arrays do.  Assigning an array uploads it
+    and assigning a scalar fills the field (state.height .= 1.0)."""
 
     _names = dict(FIELDS, **{"h∇px": FIELDS["hgradpx"], "h∇py": FIELDS["hgradpy"]})
 
@@ -95,6 +100,7 @@ class HipState:
             raise TypeError("T must be Float64 or Float32")
         if kind not in ("simple", "thermal"):
             raise ValueError("kind must be 'simple' or 'thermal'")
+        self._init_fields()
         object.__setattr__(self, "Lx", int(Lx))
         object.__setattr__(self, "Ly", int(Ly))
         object.__setattr__(self, "T", T)
@@ -119,69 +125,51 @@ class HipState:
             pass
 
     @property
-    def handle(self):
-        return self._h
-
-    @property
     def dtype(self) -> int:
         return _lib.F64 if self.T is np.float64 else _lib.F32
 
-    def _shape(self, name):
+    def _field_shape(self, name):
         return (self.Lx, self.Ly, 9) if name in DIST_FIELDS else (self.Lx, self.Ly)
 
-    def get(self, name: str, dtype=None) -> np.ndarray:
-        """Download a field; FP32 states can return FP64 (decoded exactly)."""
-        fid = self._names[name]
-        canon = [k for k, v in FIELDS.items() if v == fid][0]
-        dt = np.dtype(dtype or self.T)
-        out = np.empty(self._shape(canon), dtype=dt, order="F")
-        check(lib().swalbe_download(self._h, fid, out.ctypes.data, out.nbytes), f"download {name}")
+    def _download(self, name: str, dtype=None) -> np.ndarray:
+        out = np.empty(self._field_shape(name), dtype=np.dtype(dtype or self.T), order="F")
+        check(lib().swalbe_download(self._h, self._names[name], out.ctypes.data, out.nbytes),
+              f"download {name}")
         return out
 
-    def set(self, name: str, value) -> None:
+    def _upload(self, name: str, value) -> None:
         fid = self._names[name]
-        canon = [k for k, v in FIELDS.items() if v == fid][0]
         if np.ndim(value) == 0:
             check(lib().swalbe_fill(self._h, fid, float(value)), f"fill {name}")
             return
         arr = np.asarray(value)
-        shape = self._shape(canon)
+        shape = self._field_shape(name)
         if arr.shape != shape:
             arr = np.broadcast_to(arr, shape)
         dt = np.float64 if arr.dtype != np.float32 else np.float32
         arr = np.asfortranarray(arr, dtype=dt)
         check(lib().swalbe_upload(self._h, fid, arr.ctypes.data, arr.nbytes), f"upload {name}")
 
-    def __getattr__(self, name):
-        if name in HipState._names:
-            return self.get(name)
-        raise AttributeError(name)
-
-    def __setattr__(self, name, value):
-        if name in HipState._names:
-            self.set(name, value)
-        else:
-            object.__setattr__(self, name, value)
-
     def synchronize(self):
-        check(lib().swalbe_synchronize(self._h), "synchronize")
+        check(lib().swalbe_synchronize(self.handle), "synchronize")
 
     def set_reference_height(self, href: float, g: float = 0.0):
         """FP32 storage offset (deviation encoding); no effect for FP64."""
-        check(lib().swalbe_set_reference_height(self._h, float(href), float(g)), "set_reference_height")
+        check(lib().swalbe_set_reference_height(self.handle, float(href), float(g)),
+              "set_reference_height")
 
     def copy(self, dst: str, src: str) -> None:
         """Device-side `dst .= src` between fields of the same kind."""
-        check(lib().swalbe_copy(self._h, self._names[dst], self._names[src]), "copy")
+        check(lib().swalbe_copy(self.handle, self._names[dst], self._names[src]), "copy")
 
     def init_height(self, init) -> None:
         """Initial height from a FilmInit spec (the same kernel a batch film
         is initialised with)."""
-        check(lib().swalbe_init_height(self._h, C.byref(init)), "init_height")
+        check(lib().swalbe_init_height(self.handle, C.byref(init)), "init_height")
 
     def reduce(self, name: str = "height", thresh: float = 0.0):
         out = (C.c_double * 4)()
-        check(lib().swalbe_reduce(self._h, self._names[name], float(thresh), out), "reduce")
+        check(lib().swalbe_reduce(self.handle, self._names[name], float(thresh), out), "reduce")
         return {"sum": out[0], "max": out[1], "min": out[2], "count": int(out[3])}
 
 

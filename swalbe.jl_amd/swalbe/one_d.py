@@ -22,6 +22,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import FIELDS_1D, check, lib
+from .fields import LiveFields
 
 DIST_1D = ("fout", "ftemp", "feq")
 
@@ -74,12 +75,13 @@ def obslist(sys: SysConstWithBound_1D, T=np.float64, verbose: bool = False) -> N
 # ---------------------------------------------------------------- state
 
 
-class HipState1D:
+class HipState1D(LiveFields):
     """A 1D film resident on the GPU: the arrays of State_1D plus kbt
     (State_thermal_1D), γ and ∇γ (State_gamma_1D), θ and the wall masks.
     `state.basestate` is the state itself, so reference code written as
-    state.basestate.height reads the same.  Reading a field downloads it,
-    assigning an array uploads it, assigning a scalar fills it."""
+    state.basestate.height reads the same.  `state.height` is a live view of
+    the device array (reads see the device, in-place writes reach it before
+    the next op); assigning an array uploads it, assigning a scalar fills it."""
 
     _names = dict(FIELDS_1D, **{"h∇p": FIELDS_1D["hgradp"], "γ": FIELDS_1D["gamma"],
                                 "∇γ": FIELDS_1D["gradgamma"], "θ": FIELDS_1D["theta"]})
@@ -92,6 +94,7 @@ class HipState1D:
             raise TypeError("T must be Float64 or Float32")
         if kind not in self._kinds:
             raise ValueError(f"kind must be one of {self._kinds}")
+        self._init_fields()
         object.__setattr__(self, "L", int(L))
         object.__setattr__(self, "T", T)
         object.__setattr__(self, "kind", kind)
@@ -117,10 +120,6 @@ class HipState1D:
             pass
 
     @property
-    def handle(self):
-        return self._h
-
-    @property
     def basestate(self):
         return self
 
@@ -128,45 +127,34 @@ class HipState1D:
     def dtype(self) -> int:
         return _lib.F64 if self.T is np.float64 else _lib.F32
 
-    def _shape(self, fid):
-        return (self.L, 3) if fid <= FIELDS_1D["feq"] else (self.L,)
+    def _field_shape(self, name):
+        return (self.L, 3) if self._names[name] <= FIELDS_1D["feq"] else (self.L,)
 
-    def get(self, name: str, dtype=None) -> np.ndarray:
-        fid = self._names[name]
-        out = np.empty(self._shape(fid), dtype=np.dtype(dtype or self.T), order="F")
-        check(lib().swalbe1d_download(self._h, fid, out.ctypes.data, out.nbytes), f"download {name}")
+    def _download(self, name: str, dtype=None) -> np.ndarray:
+        out = np.empty(self._field_shape(name), dtype=np.dtype(dtype or self.T), order="F")
+        check(lib().swalbe1d_download(self._h, self._names[name], out.ctypes.data, out.nbytes),
+              f"download {name}")
         return out
 
-    def set(self, name: str, value) -> None:
+    def _upload(self, name: str, value) -> None:
         fid = self._names[name]
         if np.ndim(value) == 0:
             check(lib().swalbe1d_fill(self._h, fid, float(value)), f"fill {name}")
             return
         arr = np.asarray(value)
-        shape = self._shape(fid)
+        shape = self._field_shape(name)
         if arr.shape != shape:
             arr = np.broadcast_to(arr, shape)
         arr = np.asfortranarray(arr, dtype=np.float32 if arr.dtype == np.float32 else np.float64)
         check(lib().swalbe1d_upload(self._h, fid, arr.ctypes.data, arr.nbytes), f"upload {name}")
 
-    def __getattr__(self, name):
-        if name in HipState1D._names:
-            return self.get(name)
-        raise AttributeError(name)
-
-    def __setattr__(self, name, value):
-        if name in HipState1D._names:
-            self.set(name, value)
-        else:
-            object.__setattr__(self, name, value)
-
     def synchronize(self):
-        check(lib().swalbe1d_synchronize(self._h), "synchronize")
+        check(lib().swalbe1d_synchronize(self.handle), "synchronize")
 
     def reduce(self, name: str = "height", thresh: float = 0.0, interior: bool = False):
         out = (C.c_double * 4)()
-        check(lib().swalbe1d_reduce(self._h, self._names[name], float(thresh), int(interior), out),
-              "reduce")
+        check(lib().swalbe1d_reduce(self.handle, self._names[name], float(thresh), int(interior),
+                                    out), "reduce")
         return {"sum": out[0], "max": out[1], "min": out[2], "count": int(out[3])}
 
 
@@ -556,7 +544,7 @@ def run_flat(sys: SysConst_1D, verbos: bool = True, T=np.float64):
     """run_flat(sys::SysConst_1D) (src/simulate.jl:247-256)."""
     print("Simulating a flat interface without driving forces (nothing should happen) in one dimension")
     state = Sys(sys, T=T)
-    state.height = 1.0
+    state.height[...] = 1.0
     time_loop(sys, state, verbose=verbos)
     return state.height
 
@@ -566,9 +554,7 @@ def run_random(sys: SysConst_1D, h0: float = 1.0, eps: float = 0.01, verbos: boo
     """run_random(sys::SysConst_1D; h₀, ϵ) (src/simulate.jl:296-304)."""
     print("Simulating a random undulated interface in one dimension")
     state = Sys(sys, T=T)
-    h = np.ones(sys.L)
-    randinterface(h, h0, eps, seed=seed)
-    state.height = h
+    randinterface(state.height, h0, eps, seed=seed)
     equilibrium(state, sys)
     time_loop(sys, state, 1 / 9, verbose=verbos)
     return state.height
@@ -580,8 +566,8 @@ def run_rayleightaylor(sys: SysConst_1D, k: int = 15, h0: float = 1.0, eps: floa
     h = h₀ (1 + ϵ sin(2π k i / (L - 1))), i = 1..L; returns (height, Δh)."""
     print("Simulating the Rayleigh Taylor instability in one dimension")
     state = Sys(sys, T=T)
-    i = np.arange(1, sys.L + 1, dtype=float)
-    state.height = h0 * (1 + eps * np.sin(2 * np.pi * k * i / (sys.L - 1)))
+    for i in range(1, sys.L + 1):
+        state.height[i - 1] = h0 * (1 + eps * math.sin(2 * math.pi * k * i / (sys.L - 1)))
     diff = []
     equilibrium(state, sys)
     time_loop(sys, state, diff, verbose=verbos)
@@ -594,7 +580,7 @@ def run_dropletrelax(sys: SysConst_1D, radius=20, θ0=1 / 6, center=None, verbos
     print("Simulating an out of equilibrium droplet in one dimensions")
     area = []
     state = Sys(sys, T=T)
-    state.height = singledroplet(np.ones(sys.L), radius, θ0, sys.L // 2 if center is None else center)
+    singledroplet(state.height, radius, θ0, sys.L // 2 if center is None else center)
     equilibrium(state, sys)
     time_loop(sys, state, wetted, area, verbose=verbos)
     return state.height, area
@@ -606,7 +592,7 @@ def run_dropletpatterned(sys: SysConst_1D, radius=20, θ0=1 / 6, center=None, θ
     print("Simulating a droplet on a patterned substrate in one dimension")
     θs = np.full(sys.L, 1 / 2) if θs is None else θs
     state = Sys(sys, T=T)
-    state.height = singledroplet(np.ones(sys.L), radius, θ0, sys.L // 2 if center is None else center)
+    singledroplet(state.height, radius, θ0, sys.L // 2 if center is None else center)
     equilibrium(state, sys)
     time_loop(sys, state, θs, verbose=verbos)
     return state.height
@@ -618,7 +604,7 @@ def run_dropletforced(sys: SysConst_1D, radius=20, θ0=1 / 6, center=None, θs=N
     inclination!(f, state) each step; returns (height, vel)."""
     print("Simulating a sliding droplet in one dimension")
     state = Sys(sys, T=T)
-    state.height = singledroplet(np.ones(sys.L), radius, θ0, sys.L // 2 if center is None else center)
+    singledroplet(state.height, radius, θ0, sys.L // 2 if center is None else center)
     equilibrium(state, sys)
     print("Starting the lattice Boltzmann time loop")
     time_loop(sys, state, inclination, f, verbose=verbos)
