@@ -11,6 +11,7 @@
 // FP64 evaluates every expression in the reference's order (the library is
 // built with -ffp-contract=off); FP32 stores plain values.
 #include <atomic>
+#include <mutex>
 #include <string.h>
 
 #include <algorithm>
@@ -693,7 +694,8 @@ __global__ void k1_zero_u64(unsigned long long* p, int64_t n) {
 struct swalbe_film1d {
   int L = 0, dtype = SWALBE_F64, kind = SWALBE_SIMPLE, device = 0;
   size_t esz = 8;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // one of the device's shared 1D streams (never destroyed here)
+  void* slab = nullptr;          // one allocation holding every fixed plane below and `red`
   void* f[2] = {nullptr, nullptr};  // distribution buffers (3 planes of L)
   int fout_buf = 0, ftemp_buf = 1;  // equal after a fused loop (fout .= ftemp)
   void* feq = nullptr;
@@ -770,6 +772,34 @@ int fill1(swalbe_film1d* s, void* p, double v, int64_t n) {
   SW_BOTH1(s, hipLaunchKernelGGL(k1_fill<T>, dim3(blocks1d(n)), dim3(kNT1), 0, s->stream, (T*)p,
                                  (T)v, n));
   SW_HIP1(hipGetLastError());
+  return SWALBE_OK;
+}
+
+// Streams of the 1D films: a few per device, created on first use and kept
+// until the process ends.  A film takes one of them round-robin.  Films used
+// to own a stream each; with thousands of live films (ensembles of scan
+// films) that is thousands of HIP streams, each holding runtime queue
+// resources (signals, kernel-argument pools) for as long as the film lives.
+constexpr int kStreams1D = 4;
+constexpr int kMaxDevices1D = 64;
+std::mutex g_streams_mu;
+hipStream_t g_streams[kMaxDevices1D][kStreams1D] = {};
+unsigned g_stream_next[kMaxDevices1D] = {};
+
+int film_stream(int dev, hipStream_t* out) {
+  if (dev < 0 || dev >= kMaxDevices1D) return bad1("swalbe1d_create: device index out of range");
+  std::lock_guard<std::mutex> lk(g_streams_mu);
+  if (!g_streams[dev][0]) {
+    for (int k = 0; k < kStreams1D; ++k) {
+      const hipError_t e = hipStreamCreateWithFlags(&g_streams[dev][k], hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        for (int j = 0; j < k; ++j) (void)hipStreamDestroy(g_streams[dev][j]);
+        for (int j = 0; j < kStreams1D; ++j) g_streams[dev][j] = nullptr;
+        return hip_fail(e, "swalbe1d_create: stream creation");
+      }
+    }
+  }
+  *out = g_streams[dev][g_stream_next[dev]++ % kStreams1D];
   return SWALBE_OK;
 }
 
@@ -913,28 +943,46 @@ int swalbe1d_create(int32_t L, int32_t dtype, int32_t kind, swalbe_film1d** out)
   if (!out || L < 3 || (dtype != SWALBE_F64 && dtype != SWALBE_F32) ||
       (kind != SWALBE_SIMPLE && kind != SWALBE_THERMAL))
     return bad1("swalbe1d_create: bad arguments (L >= 3, dtype, kind)");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "swalbe1d_create: hipGetDevice");
   swalbe_film1d* s = new swalbe_film1d();
   s->L = L;
   s->dtype = dtype;
   s->kind = kind;
   s->esz = dtype == SWALBE_F64 ? 8 : 4;
-  (void)hipGetDevice(&s->device);
-  int rc = SWALBE_OK;
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-    s->stream = nullptr;
-    rc = bad1("swalbe1d_create: stream creation failed");
-  }
+  s->device = dev;
+  int rc = film_stream(dev, &s->stream);
+  // every fixed plane in one allocation, each plane 256-byte aligned
   void** planes3[] = {&s->f[0], &s->f[1], &s->feq, &s->fo};
-  for (void** p : planes3)
-    if (rc == SWALBE_OK) rc = alloc1(s, p, 3 * (int64_t)L);
   void** planes1[] = {&s->hbuf[0], &s->hbuf[1], &s->ubuf[0], &s->ubuf[1], &s->pressure,
                       &s->F,       &s->slip,    &s->hgp,     &s->kbt};
-  for (void** p : planes1)
-    if (rc == SWALBE_OK) rc = alloc1(s, p, L);
-  if (rc == SWALBE_OK && hipMalloc(&s->red, sizeof(double) * 4) != hipSuccess) {
-    s->red = nullptr;
-    set_error("swalbe1d_create: device allocation failed");
-    rc = SWALBE_ENOMEM;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b3 = up(3 * (size_t)L * s->esz), b1 = up((size_t)L * s->esz);
+  const size_t total = 4 * b3 + 9 * b1 + up(sizeof(double) * 4);
+  if (rc == SWALBE_OK) {
+    e = hipMalloc(&s->slab, total);
+    if (e != hipSuccess) {
+      s->slab = nullptr;
+      set_error(std::string("swalbe1d_create: device allocation failed: ") + hipGetErrorString(e));
+      rc = SWALBE_ENOMEM;
+    }
+  }
+  if (rc == SWALBE_OK) {
+    char* q = static_cast<char*>(s->slab);
+    for (void** p : planes3) {
+      *p = q;
+      q += b3;
+    }
+    for (void** p : planes1) {
+      *p = q;
+      q += b1;
+    }
+    s->red = reinterpret_cast<double*>(q);
+    for (void** p : planes3)
+      if (rc == SWALBE_OK) rc = fill1(s, *p, 0.0, 3 * (int64_t)L);
+    for (void** p : planes1)
+      if (rc == SWALBE_OK) rc = fill1(s, *p, 0.0, L);
   }
   if (rc == SWALBE_OK) rc = fill1(s, s->hbuf[0], 1.0, L);  // Sys: height = ones(L)
   if (rc == SWALBE_OK && hipStreamSynchronize(s->stream) != hipSuccess)
@@ -949,17 +997,19 @@ int swalbe1d_create(int32_t L, int32_t dtype, int32_t kind, swalbe_film1d** out)
 
 int swalbe1d_destroy(swalbe_film1d* s) {
   if (!s) return SWALBE_OK;
-  (void)hipSetDevice(s->device);
-  if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void* ptrs[] = {s->f[0],    s->f[1],    s->feq,        s->fo,        s->hbuf[0],
-                  s->hbuf[1], s->ubuf[0], s->ubuf[1],    s->pressure,  s->F,
-                  s->slip,    s->hgp,     s->kbt,        s->theta,     s->kap,
-                  s->gamma,   s->grad_gamma, s->border[0], s->border[1], s->interior,
-                  s->red};
+  hipError_t e = hipSetDevice(s->device);
+  // the film's work must be done before its memory goes (the stream is shared
+  // with other films: this waits for theirs too)
+  if (e == hipSuccess && s->stream) e = hipStreamSynchronize(s->stream);
+  void* ptrs[] = {s->slab, s->theta, s->kap, s->gamma, s->grad_gamma, s->border[0],
+                  s->border[1], s->interior};
   for (void* p : ptrs)
-    if (p) (void)hipFree(p);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (p) {
+      const hipError_t ef = hipFree(p);
+      if (e == hipSuccess) e = ef;
+    }
   delete s;
+  if (e != hipSuccess) return hip_fail(e, "swalbe1d_destroy");
   return SWALBE_OK;
 }
 

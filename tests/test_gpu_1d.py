@@ -499,3 +499,52 @@ def test_errors_are_reported(S):
         S.time_loop(big, S.Sys(big), variant=2)
     with pytest.raises(S.SwalbeError):
         S.time_loop(sysc, st, grad_gamma_sign=-1)  # no ∇γ field
+
+
+_MANY_FILMS = r"""
+import sys
+import numpy as np
+sys.path[:0] = [sys.argv[1]]
+import swalbe as S
+nf, L, n = int(sys.argv[2]), 128, 200
+print(f"[many films] creating {nf} films", flush=True)
+systems, states = [], []
+for k in range(nf):
+    sy = S.SysConst_1D(L=L, param=S.Taumucs(Tmax=n, tdump=10 ** 9, γ=0.01))
+    st = S.Sys(sy)
+    i = np.arange(1, L + 1)
+    st.height = 1.0 + 0.01 * np.sin(2 * np.pi * (1 + k % 7) * i / L + 0.001 * k)
+    S.equilibrium(st, sy)
+    systems.append(sy)
+    states.append(st)
+print(f"[many films] ensemble of {nf} films, {n} steps", flush=True)
+S.ensemble_time_loop(systems, states, n, materialize=False)
+for k in (0, 1, nf // 2, nf - 1):
+    sy = S.SysConst_1D(L=L, param=S.Taumucs(Tmax=n, tdump=10 ** 9, γ=0.01))
+    with S.Sys(sy) as one:
+        i = np.arange(1, L + 1)
+        one.height = 1.0 + 0.01 * np.sin(2 * np.pi * (1 + k % 7) * i / L + 0.001 * k)
+        S.equilibrium(one, sy)
+        S.time_loop(sy, one, variant=2)
+        assert np.array_equal(one.get("height"), states[k].get("height")), k
+for st in states:
+    st.close()
+print("[many films] OK", flush=True)
+"""
+
+
+def test_ensemble_of_many_live_films(S):
+    """6000 live films in one ensemble, each equal to the same film alone.
+    The films share the library's few streams per device and take one
+    allocation each.  Runs in a child process: the parent test session is
+    not at stake if the process ends abnormally."""
+    import os
+    import subprocess
+    import sys
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "swalbe.jl_amd")
+    r = subprocess.run([sys.executable, "-c", _MANY_FILMS, pkg, "6000"], capture_output=True,
+                       text=True, timeout=240)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "[many films] OK" in r.stdout
