@@ -164,7 +164,12 @@ typedef struct swalbe_step_opts {
                               0 = cached, 1 = every output streams past the caches
                               (non-temporal), 2 = the distributions and the pressure
                               stream while h, ux, uy (the next step's input) stay cached */
-    int32_t pad_;
+    /* FP32 states: 0 (default) evaluates the disjoining pressure
+     * κ [(hmin/(h+hcrit))^n - (hmin/(h+hcrit))^m] in FP64 (the stored height
+     * exactly, an IEEE division) at nodes with h + hcrit < 4 hmin, where the
+     * film nears hmin and the term dominates the pressure; 1 evaluates it in
+     * FP32 everywhere.  Batch films always use the FP64 evaluation. */
+    int32_t no_fp64_guard;
     /* snapshot!(snap, height, t; dumping) (src/measures.jl:99-105): after every
      * step t with t % snap_every == 0 the height is copied to snap_series
      * (host, one Lx*Ly column-major block per snapshot, in step order), through
@@ -321,12 +326,6 @@ int swalbe_init_height(swalbe_state *st, const swalbe_film_init *init);
 /* free and total device memory (admission by HBM footprint) */
 int swalbe_device_memory(size_t *free_bytes, size_t *total_bytes);
 
-/* ---- measurement: the data-movement ceiling of the fused step's access
- * pattern (no reference counterpart).  Reads nread planes and writes nwrite
- * planes of an Lx x Ly FP32 film over 64 x tile_y tiles with no arithmetic;
- * returns the mean device time per launch over `iters` launches. */
-int swalbe_pattern_probe(int32_t Lx, int32_t Ly, int32_t nread, int32_t nwrite, int32_t tile_y,
-                         int32_t iters, double *ms_per_launch);
 
 /* ======================================================================
  * One-dimensional films (D1Q3, velocities c = 0, +1, -1).
@@ -363,7 +362,10 @@ typedef struct swalbe_film1d swalbe_film1d;
 /* Sys(sysc::Consts_1D; T, kind) (src/initialize.jl:587-616): all fields zero
  * except height = 1.  kind SWALBE_SIMPLE or SWALBE_THERMAL ("gamma" and
  * "gamma_bound" are a simple film plus the optional γ / wall fields).
- * L >= 3. */
+ * L >= 3.  A film takes one device allocation for its fixed fields and
+ * enqueues on one of a few HIP streams the library keeps per device and
+ * shares between films (so thousands of live films cost no per-film runtime
+ * queues); synchronising a film waits for the other films on its stream. */
 int swalbe1d_create(int32_t L, int32_t dtype, int32_t kind, swalbe_film1d **out);
 int swalbe1d_destroy(swalbe_film1d *st);
 int swalbe1d_length(swalbe_film1d *st, int32_t *L);

@@ -790,6 +790,7 @@ StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, i
   sp.read_ft = sp.omeg != 0.0;
   sp.materialize = o && o->materialize;
   sp.wet_thresh = o ? o->wet_thresh : 0.0;
+  sp.fp64_guard = o ? !o->no_fp64_guard : 1;
   return sp;
 }
 

@@ -49,6 +49,36 @@ __device__ __forceinline__ float powlaw_f(float a, int n, int m, int mode) {
   return tn - tm;
 }
 
+__device__ __forceinline__ double powlaw_d(double a, int n, int m, int mode) {
+  if (mode == 1) {
+    const double t = (a * a) * a;
+    return (t * t - 1.0) * t;
+  }
+  if (mode == 2) return (a - 1.0) * (a * a);
+  double tn = 1.0, tm = 1.0;
+  for (int i = 0; i < n; ++i) tn *= a;
+  for (int i = 0; i < m; ++i) tm *= a;
+  return tn - tm;
+}
+
+// The disjoining part of the film pressure, -γ κ [a^n - a^m] with
+// a = hmin / (h + hcrit), at a node of stored height deviation dh.  Where the
+// film nears hmin (h < hguard) the term dominates the pressure and is
+// evaluated in FP64 from the stored height with an IEEE division; elsewhere
+// in FP32 with v_rcp_f32.  kap is κ (a θ field's value, or the film's).
+__device__ __forceinline__ float disjoining_f(const F32K& K, float dh, float kap, bool kap_field) {
+#pragma clang fp contract(fast)
+  const float hh = K.href + dh;
+  if (K.guard && hh < K.hguard) {
+    const double hd = K.href_d + (double)dh;
+    const double a = K.hmin_d / (hd + K.hcrit_d);
+    const double kd = kap_field ? (double)kap : K.kap_d;
+    return (float)(-K.gamma_d * (kd * powlaw_d(a, K.n, K.m, K.powmode)));
+  }
+  const float a = K.hmin * __builtin_amdgcn_rcpf(hh + K.hcrit);
+  return -K.gamma * (kap * powlaw_f(a, K.n, K.m, K.powmode));
+}
+
 __device__ __forceinline__ void store_f(float* p, float v, int nts) {
   if (nts) __builtin_nontemporal_store(v, p);
   else *p = v;
@@ -61,8 +91,8 @@ struct Smem {
   static constexpr int UC = TXW + 2, UR = TY + 2, UN = UR * UC;
   float h[HN];
   float p[PN];
-  float u[UN], v[UN];
-  float a[(SRC ? 9 : 2) * UN];
+  float2 uv[UN];                // (ux, uy) of the 1-halo nodes
+  float a[(SRC ? 9 : 2) * UN];  // SRC: nine collided planes; else (Fx, Fy) pairs
 };
 
 struct TileIO {
@@ -174,9 +204,94 @@ __device__ __forceinline__ void halo_commit(const Halo<TXW, NTH, TY>& hr,
 #pragma unroll
   for (int i = 0; i < HR::NUI; ++i) {
     const int e = tid + i * NTH;
-    if (e < SM::UN) {
-      sm.u[e] = hr.u[i];
-      sm.v[e] = hr.v[i];
+    if (e < SM::UN) sm.uv[e] = make_float2(hr.u[i], hr.v[i]);
+  }
+}
+
+// Phase D of one tile: each wave takes whole 64-column rows of output nodes;
+// per node it collides the nine populations at their source nodes (SRC:
+// reads them, collided at the source in phase C), streams them (pull: f_k(x)
+// = fo_k(x - c_k)), stores them (WF) and forms h, ux, uy (and p, WF).  Stores
+// address each plane as a row base that is uniform across the wave plus the
+// lane's column, so no per-store 64-bit address arithmetic is needed.  GRAV
+// = false drops the gravity terms (g = 0: they are exactly zero).
+template <int TXW, int NTH, int TY, bool READ_FT, bool SRC, bool SERIES, bool WF, bool GRAV>
+__device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const TileRun& R, int x0,
+                                        int y0, Smem<TXW, TY, SRC>& sm, SeriesAcc& acc) {
+#pragma clang fp contract(fast)
+  constexpr int NW = NTH / 64;
+  using SM = Smem<TXW, TY, SRC>;
+  constexpr int HC = SM::HC, PC = SM::PC, UC = SM::UC, UN = SM::UN;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int Lx = R.Lx, Ly = R.Ly;
+  const int64_t FS = R.fstride;
+  const float w1 = 1.0f / 9.0f, w5 = 1.0f / 36.0f;
+  const float2* F2 = reinterpret_cast<const float2*>(sm.a);
+  for (int rc = wv; rc < TY * (TXW / 64); rc += NW) {
+    const int row = rc / (TXW / 64), cb = 64 * (rc - row * (TXW / 64));
+    const int col = lane + cb;
+    const int gyo = y0 + row;
+    if (gyo >= Ly) continue;
+    if (x0 + col >= Lx) continue;
+    // the output row's first node: uniform, so plane + rowo is a scalar base
+    // and the lane's 32-bit byte offset the only per-lane part of an address
+    const int64_t rowo = (int64_t)Lx * gyo + x0 + cb;
+    const uint32_t lo = 4u * (uint32_t)lane;
+    auto at = [&](float* plane) { return (float*)((char*)(plane + rowo) + lo); };
+    float fo[kQ];
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+      const int su = (row + 1 - cy(k)) * UC + (col + 1 - cx(k));
+      float f;
+      if constexpr (SRC) {
+        f = sm.a[k * UN + su];
+      } else {
+        const float dh = sm.h[(row + 3 - cy(k)) * HC + (col + 3 - cx(k))];
+        const float2 uvs = sm.uv[su];
+        const float2 Fs = F2[su];
+        const float u = uvs.x, v = uvs.y;
+        const float hh = K.href + dh;
+        const float usq = u * u + v * v;
+        if (k == 0) {
+          float fe = dh - (2.0f / 3.0f) * (hh * usq);
+          if constexpr (GRAV) fe -= K.c56g * (dh * (hh + K.href));
+          f = READ_FT ? K.itau * fe : fe;
+        } else {
+          // f_k = w (g0 gg + h ((4.5 a + 3) a - 1.5 |u|^2)) + cF c_k.F, a = c_k.u
+          const float a = k == 1 ? u : k == 2 ? v : k == 3 ? -u : k == 4 ? -v
+                        : k == 5 ? u + v : k == 6 ? v - u : k == 7 ? -(u + v) : u - v;
+          const float w = k <= 4 ? w1 : w5;
+          const float cF = k <= 4 ? (1.0f / 3.0f) : (1.0f / 24.0f);
+          const float Fa = k == 1 ? Fs.x : k == 2 ? Fs.y : k == 3 ? -Fs.x : k == 4 ? -Fs.y
+                         : k == 5 ? Fs.x + Fs.y : k == 6 ? Fs.y - Fs.x : k == 7 ? -(Fs.x + Fs.y)
+                         : Fs.x - Fs.y;
+          const float t = (4.5f * w) * a + 3.0f * w;
+          float fe = hh * (t * a + (-1.5f * w) * usq);
+          if constexpr (GRAV) fe += (w * K.g0) * (dh * (hh + K.href));
+          f = READ_FT ? K.itau * fe + cF * Fa : fe + cF * Fa;
+        }
+        if constexpr (READ_FT) {
+          const int sx = wrap1(x0 + col - cx(k), Lx), sy = wrap1(gyo - cy(k), Ly);
+          f += K.omeg * io.ft[k * FS + sx + (int64_t)Lx * sy];
+        }
+      }
+      fo[k] = f;
+      if constexpr (WF) store_f(at(io.f_out + k * FS), f, R.nts);
+    }
+    const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) + fo[8];
+    const float ih = __builtin_amdgcn_rcpf(K.href + s);
+    // nts 2: only f and p stream past the caches (h, u are the next step's input)
+    const int ntm = R.nts == 1 ? 1 : 0;
+    store_f(at(io.h_out), s, ntm);
+    store_f(at(io.ux_out), ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, ntm);
+    store_f(at(io.uy_out), ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, ntm);
+    if constexpr (WF) store_f(at(io.p_out), sm.p[(row + 2) * PC + (col + 2)], R.nts);
+    if constexpr (SERIES) {
+      const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (col + 3)];
+      acc.lmax = hv > acc.lmax ? hv : acc.lmax;
+      acc.lmin = hv < acc.lmin ? hv : acc.lmin;
+      acc.cnt += hv > R.wet_thresh ? 1u : 0u;
     }
   }
 }
@@ -212,11 +327,9 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
     const float hipjp = hc[-HC - 1], himjp = hc[-HC + 1], himjm = hc[HC + 1], hipjm = hc[HC - 1];
     const float L = (2.0f / 3.0f) * (((hjp + hip) + him) + hjm) +
                     (1.0f / 6.0f) * (((hipjp + himjp) + himjm) + hipjm) - (10.0f / 3.0f) * hc[0];
-    const float hh = K.href + hc[0];
-    const float a = K.hmin * __builtin_amdgcn_rcpf(hh + K.hcrit);
     float kap = K.kap;
     if (R.use_kappa) kap = io.kappa[wx(x0 - 2 + c) + (int64_t)Lx * wy(y0 - 2 + r)];
-    sm.p[e] = -K.gamma * (kap * powlaw_f(a, K.n, K.m, K.powmode)) - K.gamma * L;
+    sm.p[e] = disjoining_f(K, hc[0], kap, R.use_kappa) - K.gamma * L;
   }
   __syncthreads();
 
@@ -231,7 +344,8 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
     const float gx = (-1.0f / 3.0f) * (fip - fim) - (1.0f / 12.0f) * (((fipjp - fimjp) - fimjm) + fipjm);
     const float gy = (-1.0f / 3.0f) * (fjp - fjm) - (1.0f / 12.0f) * (((fipjp + fimjp) - fimjm) - fipjm);
     const int ui = r * UC + c;
-    const float u = sm.u[ui], v = sm.v[ui];
+    const float2 uvn = sm.uv[ui];
+    const float u = uvn.x, v = uvn.y;
     // 2h^2 + 6δh + 3δ^2 is the denominator of both slip and noise amplitude
     const float iden = __builtin_amdgcn_rcpf((2.0f * hh + K.c6d) * hh + K.c3dd);
     const float sl = K.c6mu * hh * iden;
@@ -282,8 +396,7 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
         sm.a[k * UN + ui] = f;
       }
     } else {
-      sm.a[ui] = Fx;
-      sm.a[UN + ui] = Fy;
+      reinterpret_cast<float2*>(sm.a)[ui] = make_float2(Fx, Fy);
     }
   };
   if (R.pairs) {
@@ -323,64 +436,14 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
   __syncthreads();
 
   __builtin_amdgcn_s_setprio(1);  // the stores ahead of other waves' pressure and forces
-  // ---- D: stream (gather) and moments; a wave takes whole 64-column rows
-  for (int rc = wv; rc < TY * (TXW / 64); rc += NW) {
-    const int row = rc / (TXW / 64), col = lane + 64 * (rc - row * (TXW / 64));
-    const int gxo = x0 + col;
-    const int gyo = y0 + row;
-    if (gxo >= Lx || gyo >= Ly) continue;
-    const int64_t xo = gxo + (int64_t)Lx * gyo;
-    float fo[kQ];
-#pragma unroll
-    for (int k = 0; k < kQ; ++k) {
-      const int su = (row + 1 - cy(k)) * UC + (col + 1 - cx(k));
-      float f;
-      if constexpr (SRC) {
-        f = sm.a[k * UN + su];
-      } else {
-        const float dh = sm.h[(row + 3 - cy(k)) * HC + (col + 3 - cx(k))];
-        const float hh = K.href + dh;
-        const float u = sm.u[su], v = sm.v[su];
-        const float usq = u * u + v * v;
-        const float gg = dh * (hh + K.href);
-        const float Fx = sm.a[su], Fy = sm.a[UN + su];
-        float fe, S;
-        if (k == 0) {
-          fe = (dh - K.c56g * gg) - (2.0f / 3.0f) * hh * usq;
-          S = 0.0f;
-        } else {
-          const float a = k == 1 ? u : k == 2 ? v : k == 3 ? -u : k == 4 ? -v
-                        : k == 5 ? u + v : k == 6 ? v - u : k == 7 ? -(u + v) : u - v;
-          const float w = k <= 4 ? w1 : w5;
-          fe = w * (K.g0 * gg + hh * ((4.5f * a + 3.0f) * a - 1.5f * usq));
-          const float cF = k <= 4 ? (1.0f / 3.0f) : (1.0f / 24.0f);
-          const float Fa = k == 1 ? Fx : k == 2 ? Fy : k == 3 ? -Fx : k == 4 ? -Fy
-                         : k == 5 ? Fx + Fy : k == 6 ? Fy - Fx : k == 7 ? -(Fx + Fy) : Fx - Fy;
-          S = cF * Fa;
-        }
-        f = READ_FT ? K.itau * fe + S : fe + S;
-        if constexpr (READ_FT) {
-          const int sx = wrap1(gxo - cx(k), Lx), sy = wrap1(gyo - cy(k), Ly);
-          f += K.omeg * io.ft[k * FS + sx + (int64_t)Lx * sy];
-        }
-      }
-      fo[k] = f;
-      if (io.f_out) store_f(&io.f_out[k * FS + xo], f, R.nts);
-    }
-    const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) + fo[8];
-    const float ih = __builtin_amdgcn_rcpf(K.href + s);
-    // nts 2: only f and p stream past the caches (h, u are the next step's input)
-    const int ntm = R.nts == 1 ? 1 : 0;
-    store_f(&io.h_out[xo], s, ntm);
-    store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, ntm);
-    store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, ntm);
-    if (io.p_out) store_f(&io.p_out[xo], sm.p[(row + 2) * PC + (col + 2)], R.nts);
-    if constexpr (SERIES) {
-      const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (col + 3)];
-      acc.lmax = hv > acc.lmax ? hv : acc.lmax;
-      acc.lmin = hv < acc.lmin ? hv : acc.lmin;
-      acc.cnt += hv > R.wet_thresh ? 1u : 0u;
-    }
+  // ---- D: collide, stream (gather) and moments; one copy per store set and
+  // gravity, chosen once per tile (uniform), so the loop body has no branches
+  if (io.f_out) {
+    if (K.g0 != 0.0f) phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, true, true>(io, K, R, x0, y0, sm, acc);
+    else phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, true, false>(io, K, R, x0, y0, sm, acc);
+  } else {
+    if (K.g0 != 0.0f) phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, false, true>(io, K, R, x0, y0, sm, acc);
+    else phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, false, false>(io, K, R, x0, y0, sm, acc);
   }
   __builtin_amdgcn_s_setprio(0);
 }
@@ -484,9 +547,7 @@ __device__ __forceinline__ void strip_run(const TileIO& io, const F32K& K, const
     {
       const float L = (2.0f / 3.0f) * (((h1.v + h2.m) + h2.p) + hN.v) +
                       (1.0f / 6.0f) * (((h1.m + h1.p) + hN.p) + hN.m) - (10.0f / 3.0f) * h2.v;
-      const float hh = K.href + h2.v;
-      const float a = K.hmin * __builtin_amdgcn_rcpf(hh + K.hcrit);
-      pN = row3(-K.gamma * (kap * powlaw_f(a, K.n, K.m, K.powmode)) - K.gamma * L);
+      pN = row3(disjoining_f(K, h2.v, kap, R.use_kappa) - K.gamma * L);
     }
     if (j >= y0 - 2) {
       // F(j+1) from p rows j .. j+2, then the nine post-collision values
@@ -956,6 +1017,13 @@ F32K make_f32k(const StepParams& sp, const Enc& enc) {
   K.m = sp.m;
   // the (9,3) and (3,2) laws are evaluated in factored form
   K.powmode = (sp.n == 9 && sp.m == 3) ? 1 : ((sp.n == 3 && sp.m == 2) ? 2 : 0);
+  K.guard = sp.fp64_guard ? 1 : 0;
+  K.hguard = (float)(4.0 * sp.hmin - sp.hcrit);
+  K.gamma_d = sp.gamma;
+  K.kap_d = sp.kappa;
+  K.hmin_d = sp.hmin;
+  K.hcrit_d = sp.hcrit;
+  K.href_d = (double)enc.href;
   return K;
 }
 

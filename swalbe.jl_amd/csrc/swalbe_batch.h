@@ -11,6 +11,11 @@ struct F32K {
   float gamma, kap, hmin, hcrit, c6mu, c6d, c3dd, kb26, omeg, itau, c56g, g0, href;
   float bfx, bfy;  // body force α·fac
   int n, m, powmode;
+  // FP64 disjoining pressure where the film nears hmin: nodes with
+  // h < hguard (= 4 hmin - hcrit) when guard is set
+  int guard;
+  float hguard;
+  double gamma_d, kap_d, hmin_d, hcrit_d, href_d;
 };
 
 F32K make_f32k(const StepParams& sp, const Enc& enc);

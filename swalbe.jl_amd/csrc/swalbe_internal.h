@@ -123,6 +123,7 @@ struct StepParams {
   int read_ft;
   int materialize;
   double wet_thresh;
+  int fp64_guard;  // FP32 kernels: disjoining pressure in FP64 where h + hcrit < 4 hmin
 };
 
 StepParams make_step_params(const swalbe_params* p, const swalbe_step_opts* o, int powmode);

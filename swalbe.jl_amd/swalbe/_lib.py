@@ -60,7 +60,7 @@ class StepOpts(C.Structure):
         ("wet_thresh", C.c_double), ("dh_series", C.POINTER(C.c_double)),
         ("wet_series", C.POINTER(C.c_int64)), ("mass_series", C.POINTER(C.c_double)),
         ("tile_y", C.c_int32), ("variant", C.c_int32), ("nontemporal", C.c_int32),
-        ("pad_", C.c_int32), ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
+        ("no_fp64_guard", C.c_int32), ("snap_every", C.c_int64), ("snap_series", C.POINTER(C.c_double)),
         ("maxh_series", C.POINTER(C.c_double)), ("area_series", C.POINTER(C.c_double)),
         ("energy_series", C.POINTER(C.c_double)), ("fluid_series", C.POINTER(C.c_uint8)),
         ("surf_thresh", C.c_double), ("surf_theta_field", C.c_int32), ("final_outputs_only", C.c_int32),
@@ -189,7 +189,6 @@ def lib():
     _sig(L, "swalbe_slippage_ptr", [_I, _V, _V, _V, _V, _V, _D, _D, _I, _I, _V])
     _sig(L, "swalbe_thermal_ptr", [_I, _V, _V, _V, _D, _D, _D, C.c_uint64, C.c_uint64, _I, _I, _V])
     _sig(L, "swalbe_inclination_ptr", [_I, _V, _V, _V, _D, _D, _D, _I, _I, _V])
-    _sig(L, "swalbe_pattern_probe", [_I, _I, _I, _I, _I, _I, C.POINTER(_D)])
     _sig(L, "swalbe_malloc", [C.POINTER(_V), C.c_size_t])
     _sig(L, "swalbe_free", [_V])
     _sig(L, "swalbe_memcpy_h2d", [_V, _V, C.c_size_t])

