@@ -6,9 +6,13 @@ D2Q9 on a 4096 x 4096 periodic film, FP32 storage and compute, τ = 1,
 (Philox noise keyed by seed, step, node), h0 = 1 + 0.001 N(0,1) from a seed.
 
 A "step" is one fused time step of the whole film (one kernel launch).
-W untimed warmup steps, then K timed steps bracketed by device
-synchronisation; the step kernels' time is measured with HIP events on the
-stream they are launched on.  Prints one JSON line.
+The live ceilings and about a second of untimed steps run first (the film is
+then re-initialised), then W untimed warmup steps, then K timed steps
+bracketed by device synchronisation; the step kernels' time is measured with
+HIP events on the stream they are launched on.  The same line carries the
+second half of the headline metric: continuous-batched serving of Poisson
+solve requests (solves/s, p50/p99 latency, the speed-up over one-at-a-time
+solves and its mix-bound ceiling, a 5 % FP64 sample).  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -358,13 +362,31 @@ def bench_copy(a):
                                  "bytes_per_copy": nbytes}}), flush=True)
 
 
+def probe_lib():
+    """libswalbe_probe.so: the access-pattern probe (a bench tool, built
+    beside the library by swalbe.jl_amd/build.py)."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(ROOT, "swalbe.jl_amd", "lib", "libswalbe_probe.so"))
+    L.swalbe_pattern_probe.restype = C.c_int
+    L.swalbe_pattern_probe.argtypes = [C.c_int32] * 6 + [C.POINTER(C.c_double)]
+    return L
+
+
+def pattern_ms(Lx, Ly, tile_y, iters):
+    """Mean device ms per launch of the probe reading 3 / writing 13 planes."""
+    import ctypes as C
+    ms = C.c_double()
+    rc = probe_lib().swalbe_pattern_probe(Lx, Ly, 3, 13, tile_y, iters, C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"swalbe_pattern_probe failed with status {rc}")
+    return ms.value
+
+
 def ceilings(st, tile_y):
     """What the part allows for this step, measured live beside it: a plain
     device-to-device copy of nine 4096^2 FP32 planes (GB/s, read + write) and
     the step's own access pattern with no arithmetic (read 3 / write 13 planes
     over the same 64 x tile_y tiles, every store streaming, GLUPS)."""
-    import ctypes as C
-    from swalbe import _lib
     for _ in range(3):
         st.copy("ftemp", "feq")
     st.synchronize()
@@ -374,10 +396,9 @@ def ceilings(st, tile_y):
         st.copy("ftemp", "feq")
     st.synchronize()
     copy_gbs = 2 * 9 * 4096 * 4096 * 4 * reps / (time.perf_counter() - t) / 1e9
-    ms = C.c_double()
-    _lib.check(_lib.lib().swalbe_pattern_probe(4096, 4096, 3, 13, tile_y, 50, C.byref(ms)), "probe")
+    ms = pattern_ms(4096, 4096, tile_y, 50)
     return {"copy_GBs": round(copy_gbs, 1),
-            "pattern_GLUPS": round(4096 * 4096 / (ms.value / 1e3) / 1e9, 3),
+            "pattern_GLUPS": round(4096 * 4096 / (ms / 1e3) / 1e9, 3),
             "pattern": f"read 3 / write 13 planes over 64x{tile_y} tiles, no arithmetic, "
                        "streaming stores"}
 
@@ -387,17 +408,14 @@ def bench_pattern(a):
     ux, uy), write 13 (9 f, h, ux, uy, p) over the step's 64 x 16 tiles, no
     arithmetic, every store streaming: the ceiling the step's access pattern
     allows."""
-    import ctypes as C
-    from swalbe import _lib
-    ms = C.c_double()
     L = 4096
     ty = a.tile_y or 16
-    _lib.check(_lib.lib().swalbe_pattern_probe(L, L, 3, 13, ty, a.steps, C.byref(ms)), "probe")
-    t = ms.value / 1e3
+    ms = pattern_ms(L, L, ty, a.steps)
+    t = ms / 1e3
     gbs = 64 * L * L / t / 1e9
     print(json.dumps({"metric": "data-movement ceiling of the fused step's access pattern",
                       "value": round(L * L / t / 1e9, 3), "unit": "GLUPS", "n_gpus": 1,
-                      "steps": a.steps, "warmup": 3, "ms_per_step": round(ms.value, 5),
+                      "steps": a.steps, "warmup": 3, "ms_per_step": round(ms, 5),
                       "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                       "dtype": "f32", "data": "synthetic",
                       "config": {"workload": "pattern4096_f32: read 3 / write 13 planes of "
@@ -406,49 +424,77 @@ def bench_pattern(a):
                       "achieved_GBs": round(gbs, 1)}), flush=True)
 
 
-def bench_serve(a):
-    """Continuous-batching serving of coalescence solve requests: Poisson
-    arrivals, Lx in [128, 2048] (multiples of 64), Ly = Lx/2 or Lx,
-    T in {10^3, 10^4, 5 10^4}; admission by HBM footprint.  Reports solves/s
-    and p50/p99 latency (arrival -> height on the host) beside the same
-    requests solved one at a time (fused single-film kernel), and the rel-L2
-    of a 5 % sample against FP64 solves."""
+def serving(requests=48, rate=200.0, tick=500, graph_steps=100, alone=True,
+            low_rate=2.0, low_requests=16, sat_glups=None):
+    """Config 5, continuous-batching serving of coalescence solve requests:
+    Poisson arrivals at `rate`/s, Lx in [128, 2048] (multiples of 64),
+    Ly = Lx/2 or Lx, T in {10^3, 10^4, 5 10^4}, admission by HBM footprint.
+
+    Reports solves/s and p50/p99 latency (arrival -> height on the host);
+    the same requests solved one at a time with the fused single-film kernel
+    (each solve timed on its own: state, initial height, steps, download);
+    the speed-up over that and its mix-bound ceiling; latencies at an arrival
+    rate below capacity; the op-by-op (reference step structure) estimate;
+    and the rel-L2 of a 5 % sample against FP64 solves.
+
+    The ceiling: every solve alone takes t_alone(r); the batch cannot step the
+    mix faster than sat_glups (the fused kernel's rate on a film that fills
+    the device, final outputs only), so no server beats
+    sum(t_alone) / (sum(LUP) / sat_glups) x the one-at-a-time rate."""
     import numpy as np
     import swalbe as S
     from swalbe import serve
-    reqs = serve.poisson_requests(a.requests, rate=a.rate, seed=5)
+    reqs = serve.poisson_requests(requests, rate=rate, seed=5)
     free, total = S.device_memory()
     cap = int(min(free * 0.8, 200 << 30))
-    be = serve.BatchBackend(cap, 4096, graph_steps=a.graph_steps)
+    be = serve.BatchBackend(cap, 4096, graph_steps=graph_steps)
     # warm up the kernels and the graph path on a tiny film
-    w = serve.coalescence_request(-1, 128, 64, 2 * a.graph_steps + 3, np.random.default_rng(0), 1)
-    serve.Server(be, tick=a.tick).serve([w], realtime=False)
-    srv = serve.Server(be, tick=a.tick)
+    w = serve.coalescence_request(-1, 128, 64, 2 * graph_steps + 3, np.random.default_rng(0), 1)
+    serve.Server(be, tick=tick).serve([w], realtime=False)
+    srv = serve.Server(be, tick=tick)
     done = srv.serve(reqs)
-    be.close()
     st = serve.latency_stats(done)
     lups = sum(r.nodes * r.nsteps for r in reqs)
-    out = {"metric": "solves/s of continuous-batched FP32 coalescence solves (Poisson arrivals)",
-           "value": round(st["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
-           "steps": a.requests, "warmup": 1, "ms_per_step": None,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-           "data": "synthetic (seeded Poisson arrivals, seeded two-cap films)",
-           "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
-           "makespan_s": round(st["makespan_s"], 3), "batched_glups": round(st["glups"], 3),
-           "max_resident": srv.max_resident, "peak_arena_bytes": srv.peak_bytes}
-    if not a.no_alone:
+    finite = sum(bool(np.isfinite(r.result["height"]).all()) for r in done)
+    out = {"workload": f"serve_poisson: {requests} requests, lambda={rate}/s, Lx in [128,2048] "
+                       "step 64, Ly in {Lx/2, Lx}, T in {1e3,1e4,5e4}, two tangent caps, "
+                       "seeded theta/gamma/radius (radius capped at gamma*H <= "
+                       f"{serve.MAX_GAMMA_H}, the scheme's stability limit), FP32, tau=1",
+           "finite_results": f"{finite}/{len(done)}",
+           "solves_per_s": round(st["solves_per_s"], 4), "p50_s": round(st["p50_s"], 4),
+           "p99_s": round(st["p99_s"], 4), "makespan_s": round(st["makespan_s"], 3),
+           "batched_glups": round(st["glups"], 3), "lattice_updates": lups,
+           "max_resident": srv.max_resident, "peak_arena_bytes": srv.peak_bytes,
+           "arena_bytes": cap, "tick": tick, "graph_steps": graph_steps}
+    if low_requests:
+        # latency below capacity: arrivals slower than the server's solve rate
+        low = serve.poisson_requests(low_requests, rate=low_rate, seed=6)
+        sl = serve.latency_stats(serve.Server(be, tick=tick).serve(low))
+        out["below_capacity"] = {"rate_per_s": low_rate, "requests": low_requests,
+                                 "solves_per_s": round(sl["solves_per_s"], 4),
+                                 "p50_s": round(sl["p50_s"], 4), "p99_s": round(sl["p99_s"], 4)}
+    be.close()
+    if alone:
         # the same requests one at a time, each with the fused single-film kernel
-        t = time.perf_counter()
+        t_alone = []
         for r in reqs:
+            t = time.perf_counter()
             serve.solve_alone(r, "fused")
-        alone = time.perf_counter() - t
-        out["one_at_a_time_fused"] = {"solves_per_s": round(len(reqs) / alone, 4),
-                                      "seconds": round(alone, 3),
-                                      "glups": round(lups / alone / 1e9, 3)}
-        out["speedup_vs_one_at_a_time_fused"] = round(st["solves_per_s"] * alone / len(reqs), 3)
+            t_alone.append(time.perf_counter() - t)
+        alone_s = sum(t_alone)
+        out["one_at_a_time_fused"] = {"solves_per_s": round(len(reqs) / alone_s, 4),
+                                      "seconds": round(alone_s, 3),
+                                      "glups": round(lups / alone_s / 1e9, 3)}
+        out["speedup_vs_one_at_a_time_fused"] = round(st["solves_per_s"] * alone_s / len(reqs), 3)
+        if sat_glups:
+            ceil = alone_s / (lups / (sat_glups * 1e9))
+            out["speedup_ceiling_mix_bound"] = round(ceil, 3)
+            out["ceiling_note"] = (f"sum of the solo times / the mix's lattice updates at "
+                                   f"{sat_glups:.1f} GLUPS (the fused kernel on a film that "
+                                   "fills the device, final outputs only)")
         # plain GPU solves: the reference's op-by-op step (one kernel per op,
-        # driven from the host).  Timed for 20 steps per distinct film shape and
-        # scaled by each request's step count.
+        # driven from the host), timed for 20 steps per distinct film shape and
+        # scaled by each request's step count
         per_step = {}
         for r in reqs:
             if (r.Lx, r.Ly) not in per_step:
@@ -468,14 +514,22 @@ def bench_serve(a):
         errs.append(float(np.linalg.norm(h - ref) / np.linalg.norm(ref)))
         dmass.append(float(abs(h.sum() - ref.sum()) / ref.sum()))
         dmax.append(float(abs(h.max() - ref.max()) / ref.max()))
-    out["relL2_h_vs_fp64"] = {"sample": [f"{r.Lx}x{r.Ly} T={r.nsteps}" for r in sample],
-                              "relL2": errs, "mass_rel_diff": dmass, "max_h_rel_diff": dmax}
-    out["config"] = {"workload": f"serve_poisson: {a.requests} requests, lambda={a.rate}/s, "
-                                 "Lx in [128,2048] step 64, Ly in {Lx/2, Lx}, "
-                                 "T in {1e3,1e4,5e4}, two tangent caps, tau=1",
-                     "requests": a.requests, "lattice_updates": lups, "tick": a.tick,
-                     "graph_steps": a.graph_steps, "arena_bytes": cap,
-                     "parallelism": "single GPU"}
+    out["fp64_sample"] = {"requests": [f"{r.Lx}x{r.Ly} T={r.nsteps}" for r in sample],
+                          "relL2_h": errs, "mass_rel_diff": dmass, "max_h_rel_diff": dmax,
+                          "note": "served results are the bits of the same FP32 solve alone"}
+    return out
+
+
+def bench_serve(a):
+    """Config 5 on its own line (serve_poisson): see serving()."""
+    sv = serving(a.requests, a.rate, a.tick, a.graph_steps, alone=not a.no_alone)
+    out = {"metric": "solves/s of continuous-batched FP32 coalescence solves (Poisson arrivals)",
+           "value": sv["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+           "steps": a.requests, "warmup": 1, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded Poisson arrivals, seeded two-cap films)",
+           "config": {"workload": sv.pop("workload"), "parallelism": "single GPU"}}
+    out.update(sv)
     print(json.dumps(out), flush=True)
 
 
@@ -501,6 +555,10 @@ def main():
     ap.add_argument("--nontemporal", type=int, default=2,
                     help="FP32 stores: 0 cached, 1 all streaming, 2 f and p streaming")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="default workload: skip the config-5 serving measurement")
+    ap.add_argument("--prewarm-s", type=float, default=1.0,
+                    help="untimed seconds of steps before the warmup steps")
     ap.add_argument("--no-autotune", action="store_true",
                     help="coalesce256_batch: every film in 16-row tiles")
     a = ap.parse_args()
@@ -534,14 +592,29 @@ def main():
                       n=9, m=3, hmin=0.1, hcrit=0.05, θ=1 / 9)
     sysc = S.SysConst(Lx, Ly, param)
     st = S.Sys(sysc, "GPU", T=T, kind="thermal" if thermal else "simple")
-    _lib.check(_lib.lib().swalbe_init_random(st.handle, 1, 1234, 1.0, 0.001), "init")
-    S.equilibrium(st, sysc)
+
+    def init_film():
+        _lib.check(_lib.lib().swalbe_init_random(st.handle, 1, 1234, 1.0, 0.001), "init")
+        S.equilibrium(st, sysc)
+
+    init_film()
     prm = param.cparams()
     o = _lib.StepOpts()
     o.thermal, o.seed, o.tile_y, o.variant = int(thermal), 7, a.tile_y, a.variant
     o.nontemporal = a.nontemporal
     ms = C.c_double()
     L = _lib.lib()
+    # the live ceilings, then an untimed device prewarm: the part reaches its
+    # working clocks before the warmup steps (the film is re-initialised, so
+    # the timed steps start from the same state either way)
+    ceil = ceilings(st, a.tile_y or 16) if (Lx, Ly, dt) == (4096, 4096, "f32") else None
+    pre_steps, w0 = 0, time.perf_counter()
+    while time.perf_counter() - w0 < a.prewarm_s:
+        o.t0 = pre_steps
+        _lib.check(L.swalbe_time_loop_timed(st.handle, C.byref(prm), 100, C.byref(o), C.byref(ms)))
+        pre_steps += 100
+    pre_s = time.perf_counter() - w0
+    init_film()
     # warmup
     o.t0 = 0
     _lib.check(L.swalbe_time_loop_timed(st.handle, C.byref(prm), a.warmup, C.byref(o), C.byref(ms)))
@@ -572,6 +645,7 @@ def main():
     _lib.check(L.swalbe_time_loop_timed(st.handle, C.byref(prm), a.steps, C.byref(o), C.byref(ms2)))
     st.synchronize()
     o.final_outputs_only = 0
+    fo_glups = N * a.steps / (ms2.value / 1e3) / 1e9
     h = st.get("height", np.float64)
     ok = bool(np.isfinite(h).all())
     # the same number of steps with the film handed over by the host and the
@@ -584,7 +658,7 @@ def main():
     h_back = st.get("height", np.float64)
     pcie_s = time.perf_counter() - w0
     ok = ok and bool(np.isfinite(h_back).all())
-    ceil = ceilings(st, a.tile_y or 16) if (Lx, Ly, dt) == (4096, 4096, "f32") else None
+    st.close()
     if ceil:
         ceil["frac_of_copy"] = round(achieved / ceil["copy_GBs"], 4)
         ceil["frac_of_pattern"] = round(glups / ceil["pattern_GLUPS"], 4)
@@ -612,7 +686,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_lup": bpl},
-        "final_outputs_only": {"GLUPS": round(N * a.steps / (ms2.value / 1e3) / 1e9, 3),
+        "prewarm": {"seconds": round(pre_s, 3), "steps": pre_steps,
+                    "note": "untimed steps before the warmup (the film is re-initialised "
+                            "after them); the ceilings run before them too"},
+        "final_outputs_only": {"GLUPS": round(fo_glups, 3),
                                "bytes_per_lup": 7 * esz,
                                "note": "f and p written on the last step only (opts)"},
         "ceilings": ceil,
@@ -621,6 +698,11 @@ def main():
                           "note": "height uploaded, the same steps, height downloaded (PCIe "
                                   "both ways, wall clock)"},
     }
+    if not a.no_serving and (Lx, Ly, dt) == (4096, 4096, "f32"):
+        # config 5: the second half of the headline metric (batched solves/s
+        # and p50/p99 latency)
+        line["serving"] = serving(a.requests, a.rate, a.tick, a.graph_steps,
+                                  alone=not a.no_alone, sat_glups=fo_glups)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
         line["cpu_baseline_fused"] = cpu_baseline_fused()

@@ -5,6 +5,9 @@
 Every translation unit is compiled with -ffp-contract=off (FP64 parity with
 the reference's unfused Float64 arithmetic) and without packed-FP32
 instructions; the objects are linked into swalbe.jl_amd/lib/libswalbe_hip.so.
+The access-pattern probe bench.py measures its ceiling with is a separate
+library, swalbe.jl_amd/lib/libswalbe_probe.so: it is no part of the
+library's interface.
 """
 from __future__ import annotations
 
@@ -20,8 +23,10 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(LIBDIR, "libswalbe_hip.so")
-SOURCES = ["api.hip", "ops.hip", "step.hip", "step_f32.hip", "batch.hip", "serve.hip", "probe.hip",
+SOURCES = ["api.hip", "ops.hip", "step.hip", "step_f32.hip", "batch.hip", "serve.hip",
            "d1q3.hip", "measure.hip"]
+PROBE_SOURCES = ["probe.hip"]
+PROBE_LIB = os.path.join(LIBDIR, "libswalbe_probe.so")
 ARCH = os.environ.get("SWALBE_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -48,18 +53,22 @@ def compile_one(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 4) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(compile_one, srcs))
-    tmp = LIB + ".tmp"
+def link(objs, out: str) -> None:
+    tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(jobs: int = 4) -> str:
+    os.makedirs(OBJDIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(compile_one, SOURCES + PROBE_SOURCES))
+    link(objs[:len(SOURCES)], LIB)
+    link(objs[len(SOURCES):], PROBE_LIB)
     return LIB
 
 

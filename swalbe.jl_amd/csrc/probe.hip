@@ -1,14 +1,20 @@
-// probe.hip -- data-movement ceiling of the fused step's access pattern: a
-// kernel that reads `nread` planes and writes `nwrite` planes over the same
-// 64 x TY tiles (XCD-aware order) with no arithmetic.  Calibrates how close
-// the step runs to what its traffic pattern allows on the part.  Every
-// store streams past the caches: that is the fastest way the part moves
-// this pattern (with cached stores, or with three of thirteen planes
-// cached, it moves about two thirds as much).
-#include "swalbe_internal.h"
+// probe.hip -- libswalbe_probe.so, a measurement tool of bench.py and not
+// part of the library's interface: the data-movement ceiling of the fused
+// step's access pattern.  A kernel reads `nread` planes and writes `nwrite`
+// planes over the same 64 x TY tiles (XCD-aware order) with no arithmetic,
+// which calibrates how close the step runs to what its traffic pattern
+// allows on the part.  Every store streams past the caches: that is the
+// fastest way the part moves this pattern (with cached stores, or with three
+// of thirteen planes cached, it moves about two thirds as much).
+//
+//   int swalbe_pattern_probe(int32_t Lx, int32_t Ly, int32_t nread, int32_t nwrite,
+//                            int32_t tile_y, int32_t iters, double* ms_per_launch);
+//
+// returns 0, or 2 for bad arguments, 3 when the planes cannot be allocated,
+// 4 for a HIP error; the mean device time per launch over `iters` launches.
+#include <hip/hip_runtime.h>
 
-using swalbe::hip_fail;
-using swalbe::set_error;
+#include <cstdint>
 
 namespace {
 
@@ -55,20 +61,14 @@ __global__ __launch_bounds__(256) void k_pattern(const float* __restrict__ in,
 extern "C" int swalbe_pattern_probe(int32_t Lx, int32_t Ly, int32_t nread, int32_t nwrite,
                                     int32_t tile_y, int32_t iters, double* ms_per_launch) {
   if (Lx < 1 || Ly < 1 || nread < 0 || nread > 16 || nwrite < 1 || nwrite > 16 || iters < 1 ||
-      (tile_y != 8 && tile_y != 16) || !ms_per_launch) {
-    set_error("swalbe_pattern_probe: bad arguments");
-    return SWALBE_EINVAL;
-  }
+      (tile_y != 8 && tile_y != 16) || !ms_per_launch)
+    return 2;
   const int64_t plane = (int64_t)Lx * Ly;
   float *in = nullptr, *out = nullptr;
-  if (hipMalloc(&in, sizeof(float) * plane * (nread > 0 ? nread : 1)) != hipSuccess) {
-    set_error("swalbe_pattern_probe: allocation failed");
-    return SWALBE_ENOMEM;
-  }
+  if (hipMalloc(&in, sizeof(float) * plane * (nread > 0 ? nread : 1)) != hipSuccess) return 3;
   if (hipMalloc(&out, sizeof(float) * plane * nwrite) != hipSuccess) {
     (void)hipFree(in);
-    set_error("swalbe_pattern_probe: allocation failed");
-    return SWALBE_ENOMEM;
+    return 3;
   }
   (void)hipMemset(in, 0, sizeof(float) * plane * (nread > 0 ? nread : 1));
   const int ntx = (Lx + 63) / 64, ntiles = ntx * ((Ly + tile_y - 1) / tile_y);
@@ -100,7 +100,7 @@ extern "C" int swalbe_pattern_probe(int32_t Lx, int32_t Ly, int32_t nread, int32
   (void)hipEventDestroy(e1);
   (void)hipFree(in);
   (void)hipFree(out);
-  if (e != hipSuccess) return hip_fail(e, "swalbe_pattern_probe");
+  if (e != hipSuccess) return 4;
   *ms_per_launch = ms / iters;
-  return SWALBE_OK;
+  return 0;
 }

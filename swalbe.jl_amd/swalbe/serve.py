@@ -88,14 +88,29 @@ def two_caps_film(Lx: int, Ly: int, R: float, theta: float, seed: int, precursor
                      cap1=(R, theta, cx - b, cy), cap2=(R, theta, cx + b, cy), clamp=True)
 
 
+# The reference's scheme (τ = 1, this film-pressure stencil) stays finite
+# for caps with γ·H up to about 0.43 and blows up from about 0.48, where H =
+# R (1 - cos πθ) is the cap height (checked with the FP64 oracle on two
+# tangent caps: γ = 0.02: H = 21.4 finite, 24.1 not; γ = 0.01: H = 40.2
+# finite, 53.6 not; γ = 0.005: H = 80.4 finite).  Serving requests on films
+# up to 2048 nodes wide draw caps beyond that, which return NaN in FP64 as in
+# FP32; their sphere radius is capped at γ·H = MAX_GAMMA_H.
+MAX_GAMMA_H = 0.4
+
+
 def coalescence_request(rid: int, Lx: int, Ly: int, nsteps: int, rng: np.random.Generator,
-                        seed: int, arrival: float = 0.0) -> Request:
+                        seed: int, arrival: float = 0.0,
+                        max_gamma_h: Optional[float] = None) -> Request:
     """One droplet-coalescence solve: R ~ U(0.15, 0.3) Ly, θ/π ~ U(1/18, 1/6),
-    γ ~ U(0.005, 0.02); n = 9, m = 3, hmin = 0.1, hcrit = 0.05, δ = 1, τ = 1."""
+    γ ~ U(0.005, 0.02); n = 9, m = 3, hmin = 0.1, hcrit = 0.05, δ = 1, τ = 1.
+    max_gamma_h caps R so that γ R (1 - cos πθ) stays below it (the draws
+    are the same either way, so capped and uncapped sequences pair up)."""
     from .core import Taumucs
     R = rng.uniform(0.15, 0.3) * Ly
     theta = rng.uniform(1 / 18, 1 / 6)
     gamma = rng.uniform(0.005, 0.02)
+    if max_gamma_h is not None:
+        R = min(R, max_gamma_h / (gamma * (1.0 - math.cos(math.pi * theta))))
     param = Taumucs(Tmax=nsteps, tdump=10 ** 9, γ=gamma, θ=theta, n=9, m=3, hmin=0.1,
                     hcrit=0.05, δ=1.0)
     return Request(rid, Lx, Ly, param, two_caps_film(Lx, Ly, R, theta, seed), nsteps, arrival)
@@ -113,10 +128,12 @@ def coalescence_sweep(n: int = 256, seed: int = 1, nsteps: int = 10_000) -> List
 
 
 def poisson_requests(n: int, rate: float = 200.0, seed: int = 5,
-                     lx_range=(128, 2048), steps=(1_000, 10_000, 50_000)) -> List[Request]:
+                     lx_range=(128, 2048), steps=(1_000, 10_000, 50_000),
+                     max_gamma_h: Optional[float] = MAX_GAMMA_H) -> List[Request]:
     """n solve requests with Poisson arrivals at `rate` per second: Lx a
     multiple of 64 in lx_range, Ly = Lx/2 or Lx, T drawn from `steps`,
-    seeded θ/γ/radius."""
+    seeded θ/γ/radius, the radius capped at the scheme's stability limit
+    (MAX_GAMMA_H)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -125,7 +142,8 @@ def poisson_requests(n: int, rate: float = 200.0, seed: int = 5,
         Lx = int(rng.choice(np.arange(lx_range[0], lx_range[1] + 1, 64)))
         Ly = Lx // 2 if rng.random() < 0.5 else Lx
         T = int(rng.choice(steps))
-        out.append(coalescence_request(k, Lx, Ly, T, rng, domain_seed(seed, k), arrival=t))
+        out.append(coalescence_request(k, Lx, Ly, T, rng, domain_seed(seed, k), arrival=t,
+                                       max_gamma_h=max_gamma_h))
     return out
 
 

@@ -318,3 +318,44 @@ def test_native_server_results_equal_alone(S):
     for r in done:
         ref = serve.solve_alone(r, "fused")
         assert np.array_equal(r.result["height"], ref["height"]), r.rid
+
+
+def test_serving_config5_shapes(S):
+    """Serving at the full sizes of the Poisson serving workload: 2048 x 1024
+    requests at T = 10^4 and 5 10^4, a 2048 x 2048 one at 10^3 and small ones,
+    seeded θ/γ/radius, arriving while others run.  Each served height is the
+    bits of the same request solved alone; a 5 % sample (here the 5 10^4-step
+    2048 x 1024 request) against the FP64 solve: mass, maximum height and
+    the pointwise rel-L2 within the bounds stated below."""
+    from swalbe import serve
+    rng = np.random.default_rng(44)
+    spec = [(2048, 1024, 10_000, 0.0), (128, 64, 50_000, 0.01), (2048, 1024, 50_000, 0.02),
+            (640, 640, 10_000, 0.03), (2048, 2048, 1_000, 0.5)]
+    reqs = [serve.coalescence_request(k, Lx, Ly, T, rng, serve.domain_seed(44, k), arrival=t,
+                                      max_gamma_h=serve.MAX_GAMMA_H)
+            for k, (Lx, Ly, T, t) in enumerate(spec)]
+    be = serve.BatchBackend(8 << 30, 64, graph_steps=100)
+    srv = serve.Server(be, tick=500)
+    print("[serve] config-5 shapes: 5 requests up to 2048x2048, T up to 5e4", flush=True)
+    done = srv.serve(reqs)
+    be.close()
+    assert sorted(r.rid for r in done) == list(range(len(reqs)))
+    assert srv.max_resident >= 3
+    for r in done:
+        print(f"[serve] request {r.rid} {r.Lx}x{r.Ly} T={r.nsteps}: alone", flush=True)
+        assert np.isfinite(r.result["height"]).all(), r.rid
+        ref = serve.solve_alone(r, "fused")
+        assert np.array_equal(r.result["height"], ref["height"]), r.rid
+    r = [q for q in done if q.nsteps == 50_000 and q.Lx == 2048][0]
+    print("[serve] FP64 solve of the sample request", flush=True)
+    ref = serve.solve_alone(r, "fused", T=np.float64)["height"]
+    h = r.result["height"]
+    e = relL2(h, ref)
+    dm = abs(h.sum() - ref.sum()) / ref.sum()
+    dx = abs(h.max() - ref.max()) / ref.max()
+    print(f"[serve] FP64 sample {r.Lx}x{r.Ly} T={r.nsteps}: rel-L2 h {e:.3e}, mass {dm:.2e}, "
+          f"max h {dx:.2e}", flush=True)
+    assert np.isfinite(h).all()
+    assert dm <= 1e-3
+    assert dx <= 1e-2
+    assert e <= 0.5
