@@ -499,6 +499,18 @@ int swalbe_inclination_ptr(int32_t dtype, void *Fx, void *Fy, const void *h, dou
 int swalbe_measure_ptr(int32_t dtype, const void *h, const void *theta_field, double theta,
                        int32_t Lx, int32_t Ly, double thresh, void *mask, double *out_dev,
                        void *stream);
+/* The same measurement with no device allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  The 512 block partials
+ * live in a 16 KiB device array the library holds per GPU until the process
+ * ends, so any film size fits; the call only enqueues work on `stream` (no
+ * synchronisation) and out_dev holds the result once the stream reaches it.
+ * Calls on one GPU share that array: calls that overlap in time (on
+ * different streams or host threads) race, while calls on one stream run in
+ * order. */
+int swalbe_measure_ptr_noalloc(int32_t dtype, const void *h, const void *theta_field,
+                               double theta, int32_t Lx, int32_t Ly, double thresh, void *mask,
+                               double *out_dev, void *stream);
 /* ∇f_simple!(outx, outy, f, dgrad) -- src/measures.jl:175-192 */
 int swalbe_grad_simple_ptr(int32_t dtype, void *outx, void *outy, const void *f, int32_t Lx,
                            int32_t Ly, void *stream);

@@ -8,13 +8,57 @@
 // Reductions run in a fixed order (a fixed grid, shuffles, then one thread
 // over the block partials), so repeated runs give the same bits.
 #include <algorithm>
+#include <mutex>
 
 #include "swalbe_internal.h"
 
 namespace swalbe {
+
+// The block partials of swalbe_measure_ptr_noalloc: a device array of
+// 4 x 512 doubles (16 KiB) that each GPU holds from the library's load until
+// the process ends.  Every call on one GPU uses the same array.  In host code
+// `scratch` names a host-side stand-in, never the device memory: the device
+// address of each GPU's copy comes from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[4 * 512];
+
 namespace {
 
 constexpr int kMT = 256;
+// Blocks of one measurement: at most kMaxParts, whatever the film size (the
+// kernel strides over the film), so 4 * kMaxParts doubles hold every partial.
+constexpr int kMaxParts = 512;
+static_assert(sizeof(scratch) == sizeof(double) * 4 * kMaxParts, "scratch holds every partial");
+
+int measure_parts(int64_t N) {
+  return (int)std::min<int64_t>(kMaxParts, std::max<int64_t>(1, N / 4096));
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the current GPU's scratch array, looked up once per
+// device (with that device current) and kept.
+int scratch_address(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "swalbe_measure_ptr_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("swalbe_measure_ptr_noalloc: device index beyond the scratch table");
+    return SWALBE_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                      "swalbe_measure_ptr_noalloc: hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return SWALBE_OK;
+}
 
 __device__ __forceinline__ double wsum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -157,7 +201,7 @@ int swalbe_measure_ptr(int32_t dtype, const void* h, const void* theta_field, do
     return SWALBE_EINVAL;
   }
   const int64_t N = (int64_t)Lx * Ly;
-  const int nparts = (int)std::min<int64_t>(512, std::max<int64_t>(1, N / 4096));
+  const int nparts = measure_parts(N);
   double* partials = nullptr;
   hipError_t e = hipMalloc((void**)&partials, sizeof(double) * 4 * nparts);
   if (e != hipSuccess) return hip_fail(e, "swalbe_measure_ptr");
@@ -174,6 +218,37 @@ int swalbe_measure_ptr(int32_t dtype, const void* h, const void* theta_field, do
   (void)hipFree(partials);
   if (e == hipSuccess) e = e2;
   if (e != hipSuccess) return hip_fail(e, "swalbe_measure_ptr");
+  return SWALBE_OK;
+}
+
+int swalbe_measure_ptr_noalloc(int32_t dtype, const void* h, const void* theta_field,
+                               double theta, int32_t Lx, int32_t Ly, double thresh, void* mask,
+                               double* out_dev, void* stream) {
+  if (!h || !out_dev || Lx < 1 || Ly < 1 || (dtype != SWALBE_F64 && dtype != SWALBE_F32)) {
+    set_error("swalbe_measure_ptr_noalloc: bad arguments");
+    return SWALBE_EINVAL;
+  }
+  const int64_t N = (int64_t)Lx * Ly;
+  const int nparts = measure_parts(N);
+  if (nparts > kMaxParts) {  // cannot happen: the grid is capped at kMaxParts blocks
+    set_error("swalbe_measure_ptr_noalloc: more block partials than the scratch array holds");
+    return SWALBE_EINVAL;
+  }
+  double* partials = nullptr;
+  const int rc = scratch_address(&partials);
+  if (rc != SWALBE_OK) return rc;
+  // every block writes its own 4 partials before the finishing kernel reads
+  // them, so nothing left by an earlier call needs clearing
+  const double cs = swalbe::cospi(theta);
+  if (dtype == SWALBE_F64)
+    launch_measure<double>((const double*)h, 0.0, Lx, Ly, thresh, 1, cs,
+                           (const double*)theta_field, (uint8_t*)mask, partials, nparts, out_dev,
+                           (hipStream_t)stream);
+  else
+    launch_measure<float>((const float*)h, 0.0, Lx, Ly, thresh, 1, cs, (const float*)theta_field,
+                          (uint8_t*)mask, partials, nparts, out_dev, (hipStream_t)stream);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "swalbe_measure_ptr_noalloc");
   return SWALBE_OK;
 }
 

@@ -148,6 +148,7 @@ def lib():
     _sig(L, "swalbe_init_height", [_V, C.POINTER(FilmInit)])
     _sig(L, "swalbe_measure", [_V, _D, _I, _D, _I, C.POINTER(_D), _V, C.c_size_t])
     _sig(L, "swalbe_measure_ptr", [_I, _V, _V, _D, _I, _I, _D, _V, _V, _V])
+    _sig(L, "swalbe_measure_ptr_noalloc", [_I, _V, _V, _D, _I, _I, _D, _V, _V, _V])
     _sig(L, "swalbe_grad_simple_ptr", [_I, _V, _V, _V, _I, _I, _V])
     _sig(L, "swalbe_device_memory", [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)])
     _sig(L, "swalbe_batch_create", [_I, C.c_size_t, _I, C.POINTER(_V)])
