@@ -77,9 +77,11 @@ def test_droplet_1024_fp32_and_fp64_5e4_steps(S):
     node moved by one ulp (random sign); FP32 with the FP64 disjoining
     guard; FP32 with the disjoining pressure in FP32 everywhere.  The
     FP64-vs-perturbed-FP64 distance is the configuration's own amplification
-    of a rounding-sized difference; the FP32 errors are asserted against it.
-    Bulk quantities are asserted directly: FP64 mass to rounding, FP32 mass
-    to 1e-4, the droplet's height to 1e-3 and its volume above the
+    of a rounding-sized difference: it grows by more than 1e10 over the run,
+    so the pointwise FP32 error is bounded by that envelope (FP32 grows no
+    faster than the perturbation until it saturates), not by a fixed
+    rel-L2.  Bulk quantities are asserted directly: FP64 mass to rounding,
+    FP32 mass to 1e-4, the droplet's height to 1e-3 and its volume above the
     precursor to 1e-2."""
     from swalbe import _lib
     Lx = Ly = 1024
@@ -141,6 +143,19 @@ def test_droplet_1024_fp32_and_fp64_5e4_steps(S):
         st.close()
     assert np.isfinite(h32).all()
     assert series["f32"][0] <= 1e-5
+    # the configuration is chaotic: a 1-ulp FP64 perturbation of h0 grows by
+    # more than 1e10 over the run (measured 8e-16 at 10^3 steps, 1.2e-2 at
+    # 5 10^4), so no FP32 (or FP64) solve stays pointwise close to it
+    assert series["ulp"][-1] >= 1e10 * series["ulp"][0]
+    # FP32 diverges no faster than that perturbation does, interval by
+    # interval, until it saturates (rel-L2 of 1e-2 and above)
+    for k in range(1, len(marks)):
+        if series["f32"][k - 1] >= 1e-2:
+            break
+        grow32 = series["f32"][k] / series["f32"][k - 1]
+        grow64 = series["ulp"][k] / series["ulp"][k - 1]
+        assert grow32 <= 10 * grow64, (marks[k], grow32, grow64)
+    assert series["f32"][-1] <= 0.5
     assert d64 <= 1e-10
     assert d32 <= 1e-4
     assert abs(h32.max() - h64.max()) <= 1e-3 * h64.max()
