@@ -663,7 +663,9 @@ def main():
         ceil["frac_of_copy"] = round(achieved / ceil["copy_GBs"], 4)
         ceil["frac_of_pattern"] = round(glups / ceil["pattern_GLUPS"], 4)
     line = {
-        "metric": "GLUPS (lattice updates/s) of the fused FP32 D2Q9 thin-film step at 4096^2",
+        "metric": f"GLUPS (lattice updates/s) of the fused {'FP32' if dt == 'f32' else 'FP64'} "
+                  f"D2Q9 thin-film step at {Lx}x{Ly}" if (Lx, Ly, dt) != (4096, 4096, "f32")
+                  else "GLUPS (lattice updates/s) of the fused FP32 D2Q9 thin-film step at 4096^2",
         "value": round(glups, 3),
         "unit": "GLUPS",
         "n_gpus": 1,
