@@ -319,9 +319,11 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
   auto wx = [&](int a) { return small ? wrapm(a, Lx) : wrap1(a, Lx); };
   auto wy = [&](int a) { return small ? wrapm(a, Ly) : wrap1(a, Ly); };
 
-  // ---- B: film pressure on the 2-halo
-  for (int e = tid; e < PN; e += NT) {
-    const int r = e / PC, c = e - r * PC;
+  // ---- B: film pressure on the 2-halo.  Each wave takes whole rows, one
+  // column per lane, so each LDS access of a wave is one contiguous run of a
+  // row (no bank conflicts where a flat index would wrap into the next row);
+  // the PC - TXW columns beyond them follow, a share per wave.
+  auto pnode = [&](int r, int c) {
     const float* hc = &sm.h[(r + 1) * HC + (c + 1)];
     const float hip = hc[-1], him = hc[1], hjp = hc[-HC], hjm = hc[HC];
     const float hipjp = hc[-HC - 1], himjp = hc[-HC + 1], himjm = hc[HC + 1], hipjm = hc[HC - 1];
@@ -329,7 +331,18 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
                     (1.0f / 6.0f) * (((hipjp + himjp) + himjm) + hipjm) - (10.0f / 3.0f) * hc[0];
     float kap = K.kap;
     if (R.use_kappa) kap = io.kappa[wx(x0 - 2 + c) + (int64_t)Lx * wy(y0 - 2 + r)];
-    sm.p[e] = disjoining_f(K, hc[0], kap, R.use_kappa) - K.gamma * L;
+    sm.p[r * PC + c] = disjoining_f(K, hc[0], kap, R.use_kappa) - K.gamma * L;
+  };
+  {
+    constexpr int WPR = TXW / 64, PR = SM::PR, XB = PC - TXW, LB = XB * PR;
+    constexpr int CB = (LB + NW - 1) / NW;
+    static_assert(CB <= 64, "one pass of a wave covers its share of the edge columns");
+    for (int rc = wv; rc < PR * WPR; rc += NW) {
+      const int r = rc / WPR;
+      pnode(r, lane + 64 * (rc - r * WPR));
+    }
+    const int k = wv * CB + lane;
+    if (lane < CB && k < LB) pnode(k / XB, TXW + k % XB);
   }
   __syncthreads();
 
@@ -402,9 +415,11 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
   if (R.pairs) {
     // one Philox call per vertical node pair (x, 2m), (x, 2m+1): pair row q
     // covers halo rows 2q-1 (even y) and 2q (odd y); y0 is even
-    constexpr int NPR = TY / 2 + 2;
-    for (int t = tid; t < NPR * UC; t += NT) {
-      const int q = t / UC, c = t - q * UC;
+    // rows of pairs go to the waves whole, one column per lane (contiguous
+    // LDS runs, as in phase B); the UC - TXW edge columns go to the last wave
+    constexpr int NPR = TY / 2 + 2, WPR = TXW / 64, XC = UC - TXW, LC = XC * NPR;
+    static_assert(LC <= 64, "one pass of the last wave covers the edge columns");
+    auto pair = [&](int q, int c) {
       const int re = 2 * q - 1, ro = 2 * q;
       float z[4] = {0.f, 0.f, 0.f, 0.f};
       if constexpr (THERMAL) {
@@ -418,7 +433,12 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
       }
       if (re >= 0) fnode(re, c, z[0], z[1]);
       if (ro < UR) fnode(ro, c, z[2], z[3]);
+    };
+    for (int t = wv; t < NPR * WPR; t += NW) {
+      const int q = t / WPR;
+      pair(q, lane + 64 * (t - q * WPR));
     }
+    if (wv == NW - 1 && lane < LC) pair(lane / XC, TXW + lane % XC);
   } else {
     for (int t = tid; t < UN; t += NT) {
       const int r = t / UC, c = t - r * UC;
@@ -822,11 +842,11 @@ void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser
 // tiles (shared halos) stay in one L2.  Films are found by binary search of
 // the tile prefix table; each film's parity and step number follow from the
 // device launch counter, so the launch sequence can be a captured graph.
-__global__ __launch_bounds__(NT) void k_batch_f32(const BatchFilm* __restrict__ films,
-                                                  const int* __restrict__ tstart,
-                                                  const BatchHdr* __restrict__ hdr,
-                                                  const unsigned long long* __restrict__ counter,
-                                                  int nts) {
+// (at least 4 waves per SIMD: without the hint the tile loop's state takes
+// more VGPRs than 4 waves leave; 5, the LDS limit, would spill)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_f32(
+    const BatchFilm* __restrict__ films, const int* __restrict__ tstart,
+    const BatchHdr* __restrict__ hdr, const unsigned long long* __restrict__ counter, int nts) {
   // LDS for the taller tile; a film of 8-row tiles uses its front part
   static_assert(sizeof(Smem<TX, 8, false>) <= sizeof(Smem<TX, kBatchTileY, false>),
                 "8-row tiles fit the 16-row tile's LDS");
