@@ -12,7 +12,8 @@ bracketed by device synchronisation; the step kernels' time is measured with
 HIP events on the stream they are launched on.  The same line carries the
 second half of the headline metric: continuous-batched serving of Poisson
 solve requests (solves/s, p50/p99 latency, the speed-up over one-at-a-time
-solves and its mix-bound ceiling, a 5 % FP64 sample).  Prints one JSON line.
+solves and its mix-bound ceiling, a 5 % FP64 sample), and the 256-film batch
+(one launch per step for all films).  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -108,18 +109,19 @@ def cpu_baseline_fused(seconds_target: float = 8.0):
                                       f"{dt:.1f} s, three fused OpenMP sweeps per step, {cpu_model()}"}
 
 
-def bench_batch(a):
+def batch_run(steps, warmup, graph_steps=100, autotune=True):
     """256 independent droplet-coalescence films (Lx in {128..1024} step 64,
     Ly = Lx/2, two tangent caps, per-film θ, γ, radius) stepped together by
-    the batched FP32 kernel; K timed batch steps, run as captured graphs.
-    Each film's tile height (8 or 16 rows) is the one the autotuner measured
-    fastest for its shape (the tuning runs at admission, before the timed
-    steps, and is reported)."""
+    the batched FP32 kernel; `steps` timed batch steps after `warmup`, run as
+    captured graphs.  Each film's tile height (8 or 16 rows) is the one the
+    autotuner measured fastest for its shape (the tuning runs at admission,
+    before the timed steps, and is reported)."""
+    import numpy as np
     import swalbe as S
     from swalbe import serve
-    reqs = serve.coalescence_sweep(256, seed=1, nsteps=a.warmup + a.steps)
+    reqs = serve.coalescence_sweep(256, seed=1, nsteps=warmup + steps)
     cap = sum(S.film_bytes(r.Lx, r.Ly) for r in reqs)
-    b = S.Batch(cap, 256, graph_steps=a.graph_steps, autotune=not a.no_autotune)
+    b = S.Batch(cap, 256, graph_steps=graph_steps, autotune=autotune)
     t0 = time.perf_counter()
     slots = [b.add(r.param, r.Lx, r.Ly, r.init, r.nsteps) for r in reqs]
     b.sync()
@@ -129,38 +131,47 @@ def bench_batch(a):
     for r, ty in zip(reqs, rows):
         shapes[f"{r.Lx}x{r.Ly}"] = ty
     st = b.stats()
-    b.step(a.warmup)
+    b.step(warmup)
     b.sync()
     w0 = time.perf_counter()
-    ms = b.step_timed(a.steps)
+    ms = b.step_timed(steps)
     wall = time.perf_counter() - w0
     N = st["nodes"]
-    glups = N * a.steps / (ms / 1e3) / 1e9
-    bpl = bytes_per_lup(4, True)
-    achieved = bpl * glups
-    import numpy as np
+    glups = N * steps / (ms / 1e3) / 1e9
     ok = all(bool(np.isfinite(b.get(s, "height")).all()) for s in (0, 127, 255))
     b.close()
+    return {"glups": glups, "ms": ms, "wall": wall, "nodes": N, "films": st["films"],
+            "cap": cap, "finite": ok, "rows": rows, "shapes": shapes, "admit_s": admit_s}
+
+
+def bench_batch(a):
+    """Config 4 on its own line (coalesce256_batch): see batch_run()."""
+    r = batch_run(a.steps, a.warmup, a.graph_steps, not a.no_autotune)
+    bpl = bytes_per_lup(4, True)
+    achieved = bpl * r["glups"]
     line = {
         "metric": "GLUPS of 256 batched FP32 droplet-coalescence films (one launch per step)",
-        "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(ms / a.steps, 5),
-        "wall_ms_per_step": round(wall * 1e3 / a.steps, 5), "higher_is_better": True,
+        "value": round(r["glups"], 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(r["ms"] / a.steps, 5),
+        "wall_ms_per_step": round(r["wall"] * 1e3 / a.steps, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded two-cap films)",
-        "config": {"workload": "coalesce256_batch: 256 films, Lx uniform in {128..1024} step 64, "
-                               "Ly = Lx/2, two tangent caps R ~ U(0.15,0.3) Ly, theta/pi ~ "
-                               "U(1/18,1/6), gamma ~ U(0.005,0.02), tau=1",
-                   "films": st["films"], "nodes": N, "graph_steps": a.graph_steps,
-                   "arena_bytes": cap, "parallelism": "single GPU", "finite": ok},
+        "config": {"workload": BATCH_DESC, "films": r["films"], "nodes": r["nodes"],
+                   "graph_steps": a.graph_steps, "arena_bytes": r["cap"],
+                   "parallelism": "single GPU", "finite": r["finite"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_lup": bpl},
-        "autotune": {"on": not a.no_autotune, "shapes": len(shapes),
-                     "films_rows8": rows.count(8), "films_rows16": rows.count(16),
-                     "rows_by_shape": shapes, "admission_s": round(admit_s, 3)},
+        "autotune": {"on": not a.no_autotune, "shapes": len(r["shapes"]),
+                     "films_rows8": r["rows"].count(8), "films_rows16": r["rows"].count(16),
+                     "rows_by_shape": r["shapes"], "admission_s": round(r["admit_s"], 3)},
     }
     print(json.dumps(line), flush=True)
+
+
+BATCH_DESC = ("coalesce256_batch: 256 films, Lx uniform in {128..1024} step 64, Ly = Lx/2, "
+              "two tangent caps R ~ U(0.15,0.3) Ly, theta/pi ~ U(1/18,1/6), "
+              "gamma ~ U(0.005,0.02), tau=1")
 
 
 # FP64 operations per node and step of the 1D resident kernel in the scan
@@ -557,6 +568,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="default workload: skip the config-5 serving measurement")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="default workload: skip the config-4 batch measurement")
     ap.add_argument("--prewarm-s", type=float, default=1.0,
                     help="untimed seconds of steps before the warmup steps")
     ap.add_argument("--no-autotune", action="store_true",
@@ -700,6 +713,15 @@ def main():
                           "note": "height uploaded, the same steps, height downloaded (PCIe "
                                   "both ways, wall clock)"},
     }
+    if not a.no_batch and (Lx, Ly, dt) == (4096, 4096, "f32"):
+        # config 4: 256 films in one launch per step, every step's outputs
+        # written (its own line: --workload coalesce256_batch)
+        r = batch_run(300, 20)
+        line["batch"] = {"workload": BATCH_DESC, "GLUPS": round(r["glups"], 3),
+                         "ms_per_step": round(r["ms"] / 300, 5), "steps": 300, "warmup": 20,
+                         "films": r["films"], "nodes": r["nodes"], "graph_steps": 100,
+                         "finite": r["finite"], "films_rows16": r["rows"].count(16),
+                         "admission_s": round(r["admit_s"], 3)}
     if not a.no_serving and (Lx, Ly, dt) == (4096, 4096, "f32"):
         # config 5: the second half of the headline metric (batched solves/s
         # and p50/p99 latency)

@@ -269,6 +269,37 @@ def test_batch_film_vs_fp64_oracle(S):
     assert abs(h.sum() - ost.height.sum()) <= 1e-6 * ost.height.sum()
 
 
+def test_config4_full_batch(S):
+    """Config 4 at full size: the 256 coalescence films of the batch benchmark
+    (Lx in {128, ..., 1024} step 64, Ly = Lx/2, two tangent caps, per-film
+    θ, γ and radius; 57.2 M nodes), graph-replayed for 300 steps with every
+    output written.  Every film stays finite and keeps its mass to 1e-5;
+    four films (the first, two from the middle, the last) give the bits of
+    the same film solved alone."""
+    from dataclasses import replace
+    from swalbe import serve
+    nsteps = 300
+    reqs = serve.coalescence_sweep(256, seed=1, nsteps=nsteps)
+    cap = sum(S.film_bytes(r.Lx, r.Ly) for r in reqs)
+    print(f"[batch] config 4: 256 films, {sum(r.Lx * r.Ly for r in reqs)} nodes", flush=True)
+    b = S.Batch(cap, 256, graph_steps=100)
+    slots = [b.add(r.param, r.Lx, r.Ly, r.init, r.nsteps) for r in reqs]
+    m0 = [float(b.get(s, "height").sum()) for s in slots]
+    b.step(nsteps)
+    for k, s in enumerate(slots):
+        assert b.status(s) == (nsteps, True), k
+        h = b.get(s, "height")
+        assert np.isfinite(h).all(), k
+        assert abs(float(h.sum()) - m0[k]) <= 1e-5 * m0[k], k
+    for k in (0, 85, 170, 255):
+        r = reqs[k]
+        print(f"[batch] config 4 film {k} ({r.Lx}x{r.Ly}) against the film alone", flush=True)
+        got = {name: b.get(slots[k], name) for name in FIELDS}
+        ref = serve.solve_alone(replace(r, fields=FIELDS), "fused")
+        assert_same(got, ref, f"config 4 film {k} {r.Lx}x{r.Ly}")
+    b.close()
+
+
 def test_batch_errors(S):
     from swalbe._lib import SwalbeError
     b = S.Batch(1 << 20, 2)
