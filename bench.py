@@ -37,7 +37,11 @@ WORKLOADS = {
     "flat4096_f32_nonoise": (4096, 4096, "f32", False),
     "flat4096_f64_nonoise": (4096, 4096, "f64", False),
     "droplet1024_f32": (1024, 1024, "f32", False),
+    "droplet1024_f64": (1024, 1024, "f64", False),
 }
+# workloads that start from the sessile drop of config 2 (cap sphere radius
+# 150 at the centre, θ = 1/9, precursor 0.1) instead of a perturbed flat film
+DROPLET = ("droplet1024_f32", "droplet1024_f64")
 # many-film workloads (FP32 batched kernel), the 1D coalescence scan and the copy calibration
 BATCH_WORKLOADS = ("coalesce256_batch", "serve_poisson", "copy4096_f32", "pattern4096_f32",
                    "coalesce1d_scan", "single1d_f64", "long1d_f64")
@@ -607,7 +611,12 @@ def main():
     st = S.Sys(sysc, "GPU", T=T, kind="thermal" if thermal else "simple")
 
     def init_film():
-        _lib.check(_lib.lib().swalbe_init_random(st.handle, 1, 1234, 1.0, 0.001), "init")
+        if a.workload in DROPLET:
+            st.set_reference_height(0.1)
+            st.init_height(S.film_init(S.INIT_CAP, precursor=0.1, href=0.1, clamp=True,
+                                       cap1=(150.0, 1 / 9, Lx / 2 + 0.5, Ly / 2 + 0.5)))
+        else:
+            _lib.check(_lib.lib().swalbe_init_random(st.handle, 1, 1234, 1.0, 0.001), "init")
         S.equilibrium(st, sysc)
 
     init_film()
@@ -690,8 +699,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dt,
-        "data": "synthetic (seeded Philox film h0 = 1 + 0.001 N(0,1))",
+        "data": ("synthetic (spherical cap on a 0.1 precursor)" if a.workload in DROPLET
+                 else "synthetic (seeded Philox film h0 = 1 + 0.001 N(0,1))"),
         "config": {"workload": f"{a.workload}: D2Q9 thin film "
+                               + ("sessile drop (cap radius 150, precursor 0.1), "
+                                  if a.workload in DROPLET else "")
                                + ("rupture with thermal noise, kbt=1e-6, " if thermal else "without noise, ")
                                + f"{Lx}x{Ly} periodic, theta=1/9, n=9, m=3, gamma=0.01, tau=1",
                    "Lx": Lx, "Ly": Ly, "parallelism": "single GPU",
