@@ -112,7 +112,10 @@ struct swalbe_batch {
   unsigned long long counter = 0;  // launches enqueued so far
   bool dirty = true;
   int nactive = 0;
-  int grid = 0;
+  int grid_any = 0;        // workgroups of the kernel with the τ ≠ 1 path
+  int grid_tau1 = 0;       // workgroups of the kernel for films that all have τ = 1
+  bool any_ft = false;     // a film in the uploaded table has τ ≠ 1
+  bool graph_any = false;  // the kernel the captured graph launches
   int nts = 0;
   int last_only = 0;  // films admitted with distributions / pressure written at their end only
   int autotune = 0;   // films admitted with the tile height measured fastest for their shape
@@ -188,11 +191,13 @@ int upload_table(swalbe_batch* b) {
   int* tstart = (int*)(st + b->off_tstart);
   BatchFilm* films = (BatchFilm*)(st + b->off_films);
   int nf = 0, total = 0;
+  bool any_ft = false;
   for (auto& s : b->slots) {
     if (!s.used) continue;
     tstart[nf] = total;
     films[nf] = s.d;
     total += s.d.ntiles;
+    any_ft = any_ft || s.d.read_ft;
     ++nf;
   }
   *(BatchHdr*)st = BatchHdr{nf, total};
@@ -201,6 +206,7 @@ int upload_table(swalbe_batch* b) {
   SW_HIP(hipEventRecord(b->staging_ev[r], b->stream));
   b->staging_busy[r] = true;
   b->nactive = nf;
+  b->any_ft = any_ft;
   b->dirty = false;
   return SWALBE_OK;
 }
@@ -242,16 +248,27 @@ int wait_counter(swalbe_batch* b, unsigned long long target) {
 int enqueue_steps(swalbe_batch* b, int64_t nsteps) {
   if (int rc = upload_table(b)) return rc;
   if (nsteps == 0) return SWALBE_OK;
+  // the table just enqueued decides the kernel: the one without the f_t
+  // path while every film has τ = 1
+  const bool any = b->any_ft;
+  const int grid = any ? b->grid_any : b->grid_tau1;
   int64_t done = 0;
   if (b->graph_steps > 0 && nsteps >= b->graph_steps) {
+    if (b->gexec && b->graph_any != any) {  // captured for the other kernel
+      SW_HIP(hipGraphExecDestroy(b->gexec));
+      SW_HIP(hipGraphDestroy(b->graph));
+      b->gexec = nullptr;
+      b->graph = nullptr;
+    }
     if (!b->gexec) {
       SW_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
       for (int i = 0; i < b->graph_steps; ++i)
-        launch_batch_f32(b->films(), b->tstart(), b->hdr(), b->d_counter, b->grid, b->nts,
+        launch_batch_f32(b->films(), b->tstart(), b->hdr(), b->d_counter, grid, b->nts, any,
                          b->stream);
       hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
       if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
       SW_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->graph_any = any;
     }
     while (nsteps - done >= b->graph_steps) {
       SW_HIP(hipGraphLaunch(b->gexec, b->stream));
@@ -259,7 +276,7 @@ int enqueue_steps(swalbe_batch* b, int64_t nsteps) {
     }
   }
   for (; done < nsteps; ++done)
-    launch_batch_f32(b->films(), b->tstart(), b->hdr(), b->d_counter, b->grid, b->nts,
+    launch_batch_f32(b->films(), b->tstart(), b->hdr(), b->d_counter, grid, b->nts, any,
                      b->stream);
   SW_HIP(hipGetLastError());
   b->counter += (unsigned long long)nsteps;
@@ -461,7 +478,8 @@ int swalbe_batch_create(int32_t dtype, size_t capacity_bytes, int32_t max_films,
   int cus = 0;
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
   if (cus <= 0) cus = 256;
-  b->grid = (cus * batch_blocks_per_cu() + 7) / 8 * 8;
+  b->grid_any = (cus * batch_blocks_per_cu(true) + 7) / 8 * 8;
+  b->grid_tau1 = (cus * batch_blocks_per_cu(false) + 7) / 8 * 8;
   *out = b;
   return SWALBE_OK;
 }

@@ -848,11 +848,14 @@ void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser
 // tiles (shared halos) stay in one L2.  Films are found by binary search of
 // the tile prefix table; each film's parity and step number follow from the
 // device launch counter, so the launch sequence can be a captured graph.
-// (at least 4 waves per SIMD: without the hint the tile loop's state takes
-// more VGPRs than 4 waves leave; 5, the LDS limit, would spill)
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_f32(
-    const BatchFilm* __restrict__ films, const int* __restrict__ tstart,
-    const BatchHdr* __restrict__ hdr, const unsigned long long* __restrict__ counter, int nts) {
+// ANY_FT = false is the kernel for batches whose films all have τ = 1 (the
+// host launches it only then): without the f_t path it fits 5 workgroups per
+// CU, the LDS limit; with it, at least 4 waves per SIMD (5 would spill).
+template <bool ANY_FT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ANY_FT ? 4 : 5))) void
+k_batch_f32(const BatchFilm* __restrict__ films, const int* __restrict__ tstart,
+            const BatchHdr* __restrict__ hdr, const unsigned long long* __restrict__ counter,
+            int nts) {
   // LDS for the taller tile; a film of 8-row tiles uses its front part
   static_assert(sizeof(Smem<TX, 8, false>) <= sizeof(Smem<TX, kBatchTileY, false>),
                 "8-row tiles fit the 16-row tile's LDS");
@@ -896,6 +899,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     const BatchFilm& F = films[lo];
     if (L < F.admit || L >= F.end) continue;
+    if (!ANY_FT && F.read_ft) continue;  // never resident when this kernel runs
     const unsigned long long n = L - F.admit;
     const int cur = (int)(n & 1ull);
     TileIO io;
@@ -934,7 +938,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int x0 = (local % F.ntx) * TX, y0 = (local / F.ntx) * F.ty;
     const F32K K = F.K;
     if (F.ty == 8) {
-      if (F.read_ft) {
+      if (ANY_FT && F.read_ft) {
         if (F.thermal) tile_step<TX, NT, 8, true, true, false, false>(io, K, R, x0, y0, sm8, acc);
         else tile_step<TX, NT, 8, true, false, false, false>(io, K, R, x0, y0, sm8, acc);
       } else {
@@ -943,7 +947,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       }
     } else {
       constexpr int TY = kBatchTileY;
-      if (F.read_ft) {
+      if (ANY_FT && F.read_ft) {
         if (F.thermal) tile_step<TX, NT, TY, true, true, false, false>(io, K, R, x0, y0, sm, acc);
         else tile_step<TX, NT, TY, true, false, false, false>(io, K, R, x0, y0, sm, acc);
       } else {
@@ -1121,22 +1125,27 @@ void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries s
 }
 
 void launch_batch_f32(const BatchFilm* films, const int* tstart, const BatchHdr* hdr,
-                      unsigned long long* counter, int grid, int nts, hipStream_t s) {
+                      unsigned long long* counter, int grid, int nts, bool any_ft,
+                      hipStream_t s) {
   if (kBatchStrips)
     hipLaunchKernelGGL(k_batch_strip_f32, dim3(grid), dim3(64), 0, s, films, tstart, hdr, counter,
                        nts);
+  else if (any_ft)
+    hipLaunchKernelGGL(k_batch_f32<true>, dim3(grid), dim3(NT), 0, s, films, tstart, hdr,
+                       counter, nts);
   else
-    hipLaunchKernelGGL(k_batch_f32, dim3(grid), dim3(NT), 0, s, films, tstart, hdr, counter,
-                       nts);
+    hipLaunchKernelGGL(k_batch_f32<false>, dim3(grid), dim3(NT), 0, s, films, tstart, hdr,
+                       counter, nts);
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, s, counter);
 }
 
-int batch_blocks_per_cu() {
+int batch_blocks_per_cu(bool any_ft) {
   int per_cu = 0;
   const hipError_t e =
       kBatchStrips
           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_strip_f32, 64, 0)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_f32, NT, 0);
+      : any_ft ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_f32<true>, NT, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_f32<false>, NT, 0);
   if (e != hipSuccess || per_cu < 1) per_cu = 1;
   return per_cu;
 }

@@ -60,8 +60,11 @@ struct BatchHdr {
 };
 
 // One batched step (all resident films) followed by the counter increment.
+// any_ft: some resident film has τ ≠ 1 (reads f_t); the kernel without that
+// path needs fewer registers and holds one more workgroup per CU.
 void launch_batch_f32(const BatchFilm* films, const int* tstart, const BatchHdr* hdr,
-                      unsigned long long* counter, int grid, int nts, hipStream_t s);
-int batch_blocks_per_cu();
+                      unsigned long long* counter, int grid, int nts, bool any_ft,
+                      hipStream_t s);
+int batch_blocks_per_cu(bool any_ft);
 
 }  // namespace swalbe

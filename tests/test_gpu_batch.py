@@ -167,6 +167,32 @@ def test_staggered_admission_and_slot_reuse(S):
     b.close()
 
 
+def test_kernel_class_switches_with_tau(S):
+    """Batches whose films all have τ = 1 run a kernel without the f_t path
+    (one more workgroup per CU); admitting a τ ≠ 1 film switches to the full
+    kernel and re-captures the graph, removing it switches back.  Every film
+    keeps its alone bits across both switches."""
+    specs = film_specs(S)
+    tau1, taux = specs[0], specs[4]
+    assert tau1[2].τ == 1.0 and taux[2].τ != 1.0
+    b = S.Batch(96 << 20, 8, graph_steps=16)
+    A = b.add(*_args(tau1))             # 300 steps from launch 0
+    b.step(40)
+    D = b.add(*_args(taux))             # 250 steps from launch 40: the full kernel
+    b.step(260)
+    assert b.status(D) == (250, True)
+    got_d = {name: b.get(D, name) for name in FIELDS}
+    b.remove(D)                         # τ = 1 only again
+    E = b.add(*_args(specs[5]))         # 300 steps from launch 300
+    b.step(310)
+    assert b.status(A) == (300, True) and b.status(E) == (300, True)
+    assert_same(got_d, alone(S, *taux), "tau 0.8 film")
+    for slot, sp, what in ((A, tau1, "A"), (E, specs[5], "E")):
+        got = {name: b.get(slot, name) for name in FIELDS}
+        assert_same(got, alone(S, *sp), what)
+    b.close()
+
+
 def _args(sp):
     Lx, Ly, prm, init, n = sp
     return prm, Lx, Ly, init, n
