@@ -1,7 +1,8 @@
 // probe.hip -- libswalbe_probe.so, a measurement tool of bench.py and not
 // part of the library's interface: the data-movement ceiling of the fused
 // step's access pattern.  A kernel reads `nread` planes and writes `nwrite`
-// planes over the same 64 x TY tiles (XCD-aware order) with no arithmetic,
+// planes over the same 64 x TY tiles (in block order, backwards on odd
+// launches, as the step takes them) with no arithmetic,
 // which calibrates how close the step runs to what its traffic pattern
 // allows on the part.  Every store streams past the caches: that is the
 // fastest way the part moves this pattern (with cached stores, or with three
@@ -24,9 +25,8 @@ template <int TY, int NR, int NW>
 __global__ __launch_bounds__(256) void k_pattern(const float* __restrict__ in,
                                                  float* __restrict__ out, int Lx, int Ly,
                                                  int ntx, int ntiles, int nread, int nwrite,
-                                                 int64_t plane) {
-  int tile = blockIdx.x;
-  if ((ntiles & 7) == 0) tile = (tile & 7) * (ntiles >> 3) + (tile >> 3);
+                                                 int64_t plane, int back) {
+  const int tile = back ? ntiles - 1 - (int)blockIdx.x : (int)blockIdx.x;
   const int x0 = (tile % ntx) * 64, y0 = (tile / ntx) * TY;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int gx = x0 + lane;
@@ -73,10 +73,13 @@ extern "C" int swalbe_pattern_probe(int32_t Lx, int32_t Ly, int32_t nread, int32
   (void)hipMemset(in, 0, sizeof(float) * plane * (nread > 0 ? nread : 1));
   const int ntx = (Lx + 63) / 64, ntiles = ntx * ((Ly + tile_y - 1) / tile_y);
   const bool step = nread == 3 && nwrite == 13;  // the fused step's own counts
+  int parity = 0;
   auto launch = [&]() {
+    const int back = parity;
+    parity ^= 1;
 #define SW_P(TY, R, W)                                                                      \
   hipLaunchKernelGGL((k_pattern<TY, R, W>), dim3(ntiles), dim3(256), 0, 0, in, out, Lx, Ly, \
-                     ntx, ntiles, nread, nwrite, plane)
+                     ntx, ntiles, nread, nwrite, plane, back)
     if (tile_y == 8) {
       if (step) SW_P(8, 3, 13);
       else SW_P(8, 0, 0);

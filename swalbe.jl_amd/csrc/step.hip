@@ -116,17 +116,11 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   __shared__ unsigned long long red_k[2 * (NT / 64)];
   __shared__ unsigned int red_c[NT / 64];
 
-  // XCD-aware tile order: the 8 XCDs receive workgroups round robin, so map
-  // consecutive block ids of one XCD to neighbouring tiles (shared halos in
-  // that XCD's L2).
-  // Odd steps walk each XCD's share backwards, so a step starts on the tiles
-  // whose inputs the previous step wrote last (still in L2 and the Infinity
+  // Tiles in block order (the workgroups in flight cover a narrow band of
+  // rows), backwards on odd steps, so a step starts on the tiles whose
+  // inputs the previous step wrote last (still in L2 and the Infinity
   // Cache).  Tiles are independent, so the order changes no result.
-  int tile = blockIdx.x;
-  if ((ntiles & 7) == 0) {
-    const int share = ntiles >> 3, j = tile >> 3;
-    tile = (tile & 7) * share + ((step & 1) ? share - 1 - j : j);
-  }
+  const int tile = (step & 1) ? ntiles - 1 - (int)blockIdx.x : (int)blockIdx.x;
   const int x0 = (tile % ntx) * TX;
   const int y0 = (tile / ntx) * TY;
   const int tid = threadIdx.x;
