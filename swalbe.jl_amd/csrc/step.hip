@@ -116,11 +116,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   __shared__ unsigned long long red_k[2 * (NT / 64)];
   __shared__ unsigned int red_c[NT / 64];
 
-  // Tiles in block order (the workgroups in flight cover a narrow band of
-  // rows), backwards on odd steps, so a step starts on the tiles whose
-  // inputs the previous step wrote last (still in L2 and the Infinity
-  // Cache).  Tiles are independent, so the order changes no result.
-  const int tile = (step & 1) ? ntiles - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int tile = tile_of_block(ntx, ntiles, step);
   const int x0 = (tile % ntx) * TX;
   const int y0 = (tile / ntx) * TY;
   const int tid = threadIdx.x;

@@ -764,14 +764,7 @@ __device__ __forceinline__ void single_film_step(const TileIO& io, const F32K& K
                                                  Smem<TXW, TY, SRC>& sm,
                                                  unsigned long long* red_k, unsigned int* red_c) {
   constexpr int NW = NTH / 64;
-  // Tiles in block order, so the workgroups in flight cover a narrow band
-  // of rows (vertical neighbours, ntx apart, share an XCD when ntx is a
-  // multiple of 8); odd steps walk the tiles backwards, so a step starts on
-  // the tiles whose inputs the previous step wrote last (still in L2 and
-  // the Infinity Cache).  At 4096^2: 77.9 GLUPS, against 76.8 forwards on
-  // every step and 72.7 with each XCD walking its own band of rows.  Tiles
-  // are independent, so the order changes no result.
-  const int tile = (R.step & 1) ? ntiles - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int tile = tile_of_block(ntx, ntiles, R.step);
   const int x0 = (tile % ntx) * TXW, y0 = (tile / ntx) * TY;
   SeriesAcc acc{-__builtin_inf(), __builtin_inf(), 0u};
   const bool series = ser.dh_keys || ser.wet;
@@ -861,28 +854,37 @@ k_batch_f32(const BatchFilm* __restrict__ films, const int* __restrict__ tstart,
   Smem<TX, 8, false>& sm8 = *reinterpret_cast<Smem<TX, 8, false>*>(&sm);
   const int total = hdr->total_tiles, nf = hdr->nfilms;
   const unsigned long long L = *counter;
-  // workgroup b takes tiles b, b + G, b + 2G, ... (G workgroups), so the
-  // workgroups in flight cover one contiguous band of the tile list; odd
-  // launches walk the list backwards (a step starts on the tiles whose
-  // inputs the previous step wrote last).  The film index moves one way.
+  // XCD x = b mod 8 owns list indices 8x .. 8x + 7 of every 64 (neighbours
+  // in a film row share its L2), and its G / 8 workgroups take its indices
+  // strided, so the workgroups in flight cover one band of the list; odd
+  // launches walk backwards.  The film index moves one way.
   const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b & 7, jj = b >> 3, GX = G >> 3;
+  const int nx = ((total + 63) >> 6) << 3;  // indices per XCD, tail included
   const bool back = (L & 1ull) != 0;
-  const int nmine = b < total ? (total - 1 - b) / G + 1 : 0;
+  auto index_of = [&](int k) {  // k-th index of XCD x
+    return ((k >> 3) << 6) + (x << 3) + (k & 7);
+  };
+  const int nmine = jj < nx ? (nx - 1 - jj) / GX + 1 : 0;
   SeriesAcc acc{0.0, 0.0, 0u};
   int lo = 0, nxt = 0, fst = 0;  // film of the tile, its successor's and its own first tile
-  if (nmine > 0) {
-    const int t0 = back ? total - 1 - b : b;
-    int hi = nf - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (tstart[mid] <= t0) lo = mid;
-      else hi = mid - 1;
-    }
-    nxt = lo + 1 < nf ? tstart[lo + 1] : total;
-    fst = tstart[lo];
-  }
+  bool found = false;
   for (int i = 0; i < nmine; ++i) {
-    const int tile = back ? total - 1 - b - i * G : b + i * G;
+    const int k = back ? nx - 1 - jj - i * GX : jj + i * GX;
+    const int tile = index_of(k);
+    if (tile >= total) continue;  // the last 64 indices' tail
+    if (!found) {
+      int hi = nf - 1;
+      lo = 0;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tstart[mid] <= tile) lo = mid;
+        else hi = mid - 1;
+      }
+      nxt = lo + 1 < nf ? tstart[lo + 1] : total;
+      fst = tstart[lo];
+      found = true;
+    }
     if (back) {
       while (tile < fst) fst = tstart[--lo];  // earlier film
     } else {
