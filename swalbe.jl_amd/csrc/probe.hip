@@ -1,8 +1,8 @@
 // probe.hip -- libswalbe_probe.so, a measurement tool of bench.py and not
 // part of the library's interface: the data-movement ceiling of the fused
 // step's access pattern.  A kernel reads `nread` planes and writes `nwrite`
-// planes over the same 64 x TY tiles (in block order, backwards on odd
-// launches, as the step takes them) with no arithmetic,
+// planes over the same 64 x TY tiles, in the order the step takes them
+// (swalbe::tile_of_block), with no arithmetic,
 // which calibrates how close the step runs to what its traffic pattern
 // allows on the part.  Every store streams past the caches: that is the
 // fastest way the part moves this pattern (with cached stores, or with three
@@ -17,6 +17,8 @@
 
 #include <cstdint>
 
+#include "swalbe_internal.h"
+
 namespace {
 
 // NR / NW > 0: compile-time plane counts (all loads of a row issued at
@@ -26,7 +28,7 @@ __global__ __launch_bounds__(256) void k_pattern(const float* __restrict__ in,
                                                  float* __restrict__ out, int Lx, int Ly,
                                                  int ntx, int ntiles, int nread, int nwrite,
                                                  int64_t plane, int back) {
-  const int tile = back ? ntiles - 1 - (int)blockIdx.x : (int)blockIdx.x;
+  const int tile = swalbe::tile_of_block(ntx, ntiles, (uint64_t)back);
   const int x0 = (tile % ntx) * 64, y0 = (tile / ntx) * TY;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int gx = x0 + lane;
