@@ -205,6 +205,61 @@ def test_fused_loop_variants_fp64(S, shape, variant):
         assert np.array_equal(got, ref), f"{name}: rel-L2 {relL2(got, ref):.3e}"
 
 
+@pytest.mark.parametrize("shape", [(4096, 64), (8192, 24)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_wide_films_fp64_bitwise(S, shape):
+    """Films at least 64 tiles wide take their tiles in the XCD column-group
+    order (tile_of_block: each XCD takes columns 8x .. 8x + 7 of every 64,
+    backwards on odd steps); narrower films and every other parity test use
+    plain block order.  Both must compute every tile once: FP64 bit-exact
+    against the oracle over 21 steps (odd and even), with thermal noise."""
+    Lx, Ly = shape
+    print(f"[parity] wide film {Lx}x{Ly}", flush=True)
+    nsteps = 21
+    h0 = O.init_normal(Lx, Ly, 9, 1.0, 0.01)
+    prm = O.params(gamma=0.01, kbt=1e-5, tdump=10 ** 9)
+    ost = _oracle_state(h0)
+    O.time_loop(ost, prm, nsteps, thermal=True, seed=3)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=10 ** 9, γ=0.01, kbt=1e-5))
+    st = S.Sys(sysc, "GPU", kind="thermal")
+    st.height = h0
+    S.equilibrium(st, sysc)
+    S.time_loop(sysc, st, thermal=True, seed=3)
+    for name, ref in (("height", ost.height), ("velx", ost.velx), ("vely", ost.vely),
+                      ("pressure", ost.pressure), ("fout", ost.fout)):
+        got = st.get(name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {relL2(got, ref):.3e}"
+    st.close()
+
+
+def test_wide_film_fp32_tolerance(S):
+    """The FP32 step on a film 64 tiles wide (the XCD column-group order):
+    rel-L2(h) against the FP64 oracle <= 1e-5 after 41 steps with noise."""
+    import ctypes as C
+    from swalbe import _lib
+    Lx, Ly, nsteps = 4096, 48, 41
+    print(f"[parity] wide FP32 film {Lx}x{Ly}", flush=True)
+    h0 = O.init_normal(Lx, Ly, 10, 1.0, 0.001)
+    prm = O.params(gamma=0.01, kbt=1e-6, tdump=10 ** 9)
+    ost = _oracle_state(h0)
+    O.time_loop(ost, prm, nsteps, thermal=True, seed=42)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=10 ** 9, kbt=1e-6))
+    st = S.Sys(sysc, "GPU", T=np.float32)
+    st.set_reference_height(1.0)
+    st.height = h0
+    S.equilibrium(st, sysc)
+    o = _lib.StepOpts()
+    o.thermal, o.seed = 1, 42
+    _lib.check(_lib.lib().swalbe_time_loop(st.handle, C.byref(sysc.param.cparams()), nsteps,
+                                           C.byref(o)))
+    h = st.get("height", np.float64)
+    st.close()
+    e = relL2(h, ost.height)
+    dev = relL2(h - h.mean(), ost.height - ost.height.mean())
+    print(f"[parity] wide FP32 rel-L2 h = {e:.3e}, dynamics {dev:.3e}", flush=True)
+    assert np.isfinite(h).all()
+    assert e <= 1e-5 and dev <= 1e-3
+
+
 def test_generic_callback_path(S):
     """An arbitrary f(measure, state) runs the per-op kernels between the force
     sum and equilibrium! (src/simulate.jl:84-92) and matches the fused loop."""
