@@ -303,38 +303,49 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-// One workgroup: output nodes x0 .. x0+255; h on a 3-halo, u and F on 1, p on 2.
+// Nodes per thread of k1_step: two up to 512 threads (LDS at most 49 KB).
+// Measured at L = 2^24 with 256 threads: 69.4 GLUPS at 2, 52.5 at 4 (3 waves
+// per SIMD), 62.2 at 1.
+constexpr int k1_npt(int nb) { return nb <= 512 ? 2 : 1; }
+
+// One workgroup: output nodes x0 .. x0 + NB*NPT - 1 (NPT per thread); h on a
+// 3-halo, u on 1, p on 2, and the post-collision populations of the nodes
+// x0 - 1 .. x0 + W, collided once at their source (each node's equilibrium
+// is built once, not once per population that leaves it).  Odd steps take
+// the blocks in reverse, so a step starts where the previous one stopped
+// writing (those inputs are still in the caches).  Nodes are independent:
+// neither order nor grouping changes a result.
+
 template <typename T, int NB>
 __global__ __launch_bounds__(NB) void k1_step(Film1 S, uint64_t step,
                                               unsigned long long* dh_keys,
                                               unsigned long long* wet) {
-  __shared__ T sh[NB + 6], su[NB + 2], sp[NB + 4], sF[NB + 2];
+  constexpr int NPT = k1_npt(NB), W = NB * NPT;
+  __shared__ T sh[W + 6], su[W + 2], sp[W + 4], s0[W + 2], s1[W + 2], s2[W + 2];
   const int t = threadIdx.x, L = S.L;
-  // odd steps take the blocks in reverse, so a step starts where the
-  // previous one stopped writing (those inputs are still in the caches):
-  // 61.8 against 59.9 GLUPS for L = 2^24; nodes are independent, so the
-  // order changes no result
   const int bb = (step & 1) ? (int)gridDim.x - 1 - (int)blockIdx.x : (int)blockIdx.x;
-  const int x0 = bb * NB;
+  const int x0 = bb * W;
   const T* h_in = (const T*)S.h_in;
   const T* u_in = (const T*)S.u_in;
   const T* kap_f = (const T*)S.kap;
   const T* gam_f = (const T*)S.gam;
   // a workgroup whose 3-halo wraps nowhere indexes directly (no modulo)
-  const bool inner = x0 >= 3 && x0 + NB + 3 <= L;
+  const bool inner = x0 >= 3 && x0 + W + 3 <= L;
   auto wr = [&](int y) { return inner ? y : wrapL(y, L); };
   __builtin_amdgcn_s_setprio(2);  // the loads ahead of other waves' arithmetic
-  for (int i = t; i < NB + 6; i += NB) sh[i] = h_in[wr(x0 - 3 + i)];
-  for (int i = t; i < NB + 2; i += NB) su[i] = u_in[wr(x0 - 1 + i)];
+  for (int i = t; i < W + 6; i += NB) sh[i] = h_in[wr(x0 - 3 + i)];
+  for (int i = t; i < W + 2; i += NB) su[i] = u_in[wr(x0 - 1 + i)];
   __builtin_amdgcn_s_setprio(0);
   __syncthreads();
-  for (int i = t; i < NB + 4; i += NB) {  // p at node x0 - 2 + i
+  for (int i = t; i < W + 4; i += NB) {  // p at node x0 - 2 + i
     const int y = wr(x0 - 2 + i);
     sp[i] = pressure_node(sh[i], sh[i + 1], sh[i + 2], gam_f ? gam_f[y] : (T)S.gamma,
                           kap_f ? kap_f[y] : (T)S.kap_s, S.n, S.m, (T)S.hmin, (T)S.hcrit, 0);
   }
   __syncthreads();
-  for (int i = t; i < NB + 2; i += NB) {  // F at node x0 - 1 + i
+  const T g = (T)S.g, omeg = (T)S.omeg, itau = (T)S.itau;
+  const T* ft = (const T*)S.ft;
+  for (int i = t; i < W + 2; i += NB) {  // force and collision of node x0 - 1 + i
     const int y = wr(x0 - 1 + i);
     const T hh = sh[i + 2];
     const T hgp = (hh * T(-0.5)) * (sp[i] - sp[i + 2]);
@@ -342,60 +353,61 @@ __global__ __launch_bounds__(NB) void k1_step(Film1 S, uint64_t step,
     if (S.thermal) F = F - noise_node<T>(S.seed, step, y) * noise_amp(hh, (T)S.kb26, (T)S.delta);
     if (S.gg) F = S.gg_sign < 0 ? F - ((const T*)S.gg)[y] : F + ((const T*)S.gg)[y];
     if (S.body) F = F + (hh * (T)S.alpha) * (T)S.fac;
-    sF[i] = F;
+    T e0, e1, e2;
+    feq_node(hh, su[i], g, e0, e1, e2);
+    const T t0 = S.read_ft ? ft[y] : T(0), t1 = S.read_ft ? ft[L + y] : T(0),
+            t2 = S.read_ft ? ft[2 * L + y] : T(0);
+    s0[i] = omeg * t0 + itau * e0;
+    s1[i] = (omeg * t1 + itau * e1) + T(0.5) * F;
+    s2[i] = (omeg * t2 + itau * e2) - T(0.5) * F;
   }
   __syncthreads();
-  const int x = x0 + t;
-  const bool live = x < L;
-  if (dh_keys || wet) {  // series over the step's input height, one atomic per wave
-    const double hv = (double)sh[t + 3];
-    unsigned long long kmax = live ? dkey(hv) : 0ull, kmin = live ? dkey(hv) : ~0ull;
-    unsigned long long c = live && hv > S.wet_thresh ? 1ull : 0ull;
+  const T* b1 = (const T*)S.b1;
+  const T* b2 = (const T*)S.b2;
+  T* fout = (T*)S.f_out;
+  unsigned long long kmax = 0ull, kmin = ~0ull, cnt = 0ull;
+#pragma unroll
+  for (int q = 0; q < NPT; ++q) {
+    const int j = t + q * NB;  // output node x0 + j
+    const int x = x0 + j;
+    if (x >= L) break;
+    if (dh_keys || wet) {
+      const double hv = (double)sh[j + 3];
+      const unsigned long long k = dkey(hv);
+      kmax = k > kmax ? k : kmax;
+      kmin = k < kmin ? k : kmin;
+      cnt += hv > S.wet_thresh ? 1ull : 0ull;
+    }
+    const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
+    const T f0 = s0[j + 1];
+    T f1 = s1[j], f2 = s2[j + 2];
+    if (b1) {
+      f1 = (f1 - f1 * b1[xm]) + s2[j + 1] * b2[x];
+      f2 = (f2 - f2 * b2[xp]) + s1[j + 1] * b1[x];
+    }
+    fout[x] = f0;
+    fout[L + x] = f1;
+    fout[2 * L + x] = f2;
+    const T hn = (f0 + f1) + f2;
+    ((T*)S.h_out)[x] = hn;
+    ((T*)S.u_out)[x] = (f1 - f2) / hn;
+    ((T*)S.p_out)[x] = sp[j + 2];
+  }
+  if (dh_keys || wet) {  // one atomic per wave
     for (int o = 32; o > 0; o >>= 1) {
       const unsigned long long a = __shfl_xor(kmax, o), b = __shfl_xor(kmin, o);
       kmax = a > kmax ? a : kmax;
       kmin = b < kmin ? b : kmin;
-      c += __shfl_xor(c, o);
+      cnt += __shfl_xor(cnt, o);
     }
     if ((t & 63) == 0) {
       if (dh_keys) {
         atomicMax(&dh_keys[0], kmax);
         atomicMin(&dh_keys[1], kmin);
       }
-      if (wet && c) atomicAdd(wet, c);
+      if (wet && cnt) atomicAdd(wet, cnt);
     }
   }
-  if (!live) return;
-  const T g = (T)S.g, omeg = (T)S.omeg, itau = (T)S.itau;
-  const T* ft = (const T*)S.ft;
-  const T* b1 = (const T*)S.b1;
-  const T* b2 = (const T*)S.b2;
-  // post-collision population k of node y (tile index j = y - x0 + 1)
-  auto fo = [&](int k, int j, int y) -> T {
-    T e0, e1, e2;
-    feq_node(sh[j + 2], su[j], g, e0, e1, e2);
-    const T fe = k == 0 ? e0 : (k == 1 ? e1 : e2);
-    const T tk = S.read_ft ? ft[k * L + y] : T(0);
-    T v = omeg * tk + itau * fe;
-    if (k == 1) v = v + T(0.5) * sF[j];
-    if (k == 2) v = v - T(0.5) * sF[j];
-    return v;
-  };
-  const int xm = x == 0 ? L - 1 : x - 1, xp = x == L - 1 ? 0 : x + 1;
-  const T f0 = fo(0, t + 1, x);
-  T f1 = fo(1, t, xm), f2 = fo(2, t + 2, xp);
-  if (b1) {
-    f1 = (f1 - f1 * b1[xm]) + fo(2, t + 1, x) * b2[x];
-    f2 = (f2 - f2 * b2[xp]) + fo(1, t + 1, x) * b1[x];
-  }
-  T* fout = (T*)S.f_out;
-  fout[x] = f0;
-  fout[L + x] = f1;
-  fout[2 * L + x] = f2;
-  const T hn = (f0 + f1) + f2;
-  ((T*)S.h_out)[x] = hn;
-  ((T*)S.u_out)[x] = (f1 - f2) / hn;
-  ((T*)S.p_out)[x] = sp[t + 2];
 }
 
 // Pointers read from the film table are generic to the compiler, which then
@@ -1382,7 +1394,7 @@ int loop_per_step(swalbe_film1d* s, const swalbe_params* p, int64_t nsteps,
     int64_t km = 0, ks = 0;
     // nodes (threads) per workgroup of k1_step: opts.threads, else 256
     const int nbt = o->threads ? o->threads : 256;
-    const int nb = (s->L + nbt - 1) / nbt;
+    const int nb = (s->L + nbt * k1_npt(nbt) - 1) / (nbt * k1_npt(nbt));
     for (int64_t it = 1; it <= nsteps; ++it) {
       const int64_t t = o->t0 + it;
       S.h_in = s->hbuf[c];
