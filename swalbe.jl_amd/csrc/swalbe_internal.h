@@ -17,21 +17,21 @@ namespace swalbe {
 // per row).  The workgroups in flight (the part holds about 1280) then
 // cover one narrow band of tile rows: their loads and stores fall on few
 // DRAM pages, and vertical neighbours are in flight together.  Consecutive
-// blocks go round the 8 XCDs, so where ntx is a multiple of 64 the blocks
-// of XCD x take columns 8x .. 8x + 7 of every 64: neighbours in a row share
-// that XCD's L2 (their halo columns) except every eighth.  Odd steps walk
-// backwards, so a step starts on the tiles whose inputs the previous step
-// wrote last, still in L2 and the Infinity Cache.  Tiles are independent:
-// the order changes no result.
+// blocks go round the 8 XCDs, so where ntx is a multiple of 8 the blocks of
+// XCD x take the x-th eighth of every row (ntx / 8 adjacent columns):
+// horizontal neighbours share that XCD's L2 (their halo columns) except at
+// the eighths' edges.  Odd steps walk backwards, so a step starts on the
+// tiles whose inputs the previous step wrote last, still in L2 and the
+// Infinity Cache.  Tiles are independent: the order changes no result.
 __device__ __forceinline__ int tile_of_block(int ntx, int ntiles, uint64_t step) {
   const int b = (int)blockIdx.x;
   const bool back = (step & 1) != 0;
-  if ((ntx & 63) != 0) return back ? ntiles - 1 - b : b;
+  if ((ntx & 7) != 0) return back ? ntiles - 1 - b : b;
   const int x = b & 7, J = ntiles >> 3;
   const int j = back ? J - 1 - (b >> 3) : (b >> 3);
   const int cpr = ntx >> 3;  // columns of each XCD per row of tiles
   const int row = j / cpr, k = j - row * cpr;
-  return row * ntx + (k >> 3) * 64 + x * 8 + (k & 7);
+  return row * ntx + x * cpr + k;
 }
 
 // FP32 states store height and distributions as deviations from a rest
