@@ -116,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   __shared__ unsigned long long red_k[2 * (NT / 64)];
   __shared__ unsigned int red_c[NT / 64];
 
-  const int tile = tile_of_block(ntx, ntiles, step);
+  const int tile = tile_of_block_f64(ntx, ntiles, step, 3 * (int64_t)sizeof(T) * Lx * Ly);
   const int x0 = (tile % ntx) * TX;
   const int y0 = (tile / ntx) * TY;
   const int tid = threadIdx.x;

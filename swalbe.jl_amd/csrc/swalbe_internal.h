@@ -34,6 +34,19 @@ __device__ __forceinline__ int tile_of_block(int ntx, int ntiles, uint64_t step)
   return row * ntx + x * cpr + k;
 }
 
+// The FP64 step's order for a film whose inputs (h, ux, uy) fit the eight
+// L2s (4 MB each, so in_bytes <= 32 MB) and whose tile count is a multiple
+// of 8: XCD x takes the x-th eighth of the tile list, a band of rows, and
+// reads the same band step after step from its own L2.  Odd steps walk each
+// band backwards.  Other films take tile_of_block.
+__device__ __forceinline__ int tile_of_block_f64(int ntx, int ntiles, uint64_t step,
+                                                 int64_t in_bytes) {
+  if ((ntiles & 7) != 0 || in_bytes > ((int64_t)32 << 20))
+    return tile_of_block(ntx, ntiles, step);
+  const int b = (int)blockIdx.x, share = ntiles >> 3, j = b >> 3;
+  return (b & 7) * share + ((step & 1) != 0 ? share - 1 - j : j);
+}
+
 // FP32 states store height and distributions as deviations from a rest
 // state of reference height href: dh = h - href and f'_k = f_k - R_k with
 // R = feq(href, u = 0).  FP64 states use href = 0 and R = 0 (plain values).
