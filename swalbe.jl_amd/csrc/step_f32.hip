@@ -84,15 +84,24 @@ __device__ __forceinline__ void store_f(float* p, float v, int nts) {
   else *p = v;
 }
 
-template <int TXW, int TY, bool SRC>
+// LDS of one tile: h on the 3-halo, p on the 2-halo, (ux, uy) and the phase C
+// results on the 1-halo.  AL (never with SRC): the (Fx, Fy) pairs overwrite
+// p once every wave has read it (a barrier between), so the two share one
+// buffer: 25.2 KB for a 64 x 16 tile (6 workgroups per CU) against 30.6 KB.
+template <int TXW, int TY, bool SRC, bool ALIAS = !SRC>
 struct Smem {
+  static constexpr bool AL = ALIAS;
   static constexpr int HC = TXW + 6, HR = TY + 6, HN = HR * HC;
   static constexpr int PC = TXW + 4, PR = TY + 4, PN = PR * PC;
   static constexpr int UC = TXW + 2, UR = TY + 2, UN = UR * UC;
+  static_assert(!(SRC && AL), "the collided planes never share p's buffer");
+  static constexpr int AN = SRC ? 9 * UN : AL && PN > 2 * UN ? PN : 2 * UN;
+  static_assert(HN % 2 == 0 && PN % 2 == 0, "float2 planes stay 8-byte aligned");
   float h[HN];
-  float p[PN];
-  float2 uv[UN];                // (ux, uy) of the 1-halo nodes
-  float a[(SRC ? 9 : 2) * UN];  // SRC: nine collided planes; else (Fx, Fy) pairs
+  float ps[AL ? 2 : PN];  // p, unless AL
+  float2 uv[UN];          // (ux, uy) of the 1-halo nodes
+  float a[AN];            // SRC: nine collided planes; else (Fx, Fy) pairs (AL: p first)
+  __device__ __forceinline__ float* p() { return AL ? a : ps; }
 };
 
 struct TileIO {
@@ -190,10 +199,8 @@ __device__ __forceinline__ void halo_load(const TileIO& io, const TileRun& R, in
 }
 
 // ---- A2: park the halo in LDS (the block synchronises before phase B)
-template <int TXW, int NTH, int TY, bool SRC>
-__device__ __forceinline__ void halo_commit(const Halo<TXW, NTH, TY>& hr,
-                                            Smem<TXW, TY, SRC>& sm) {
-  using SM = Smem<TXW, TY, SRC>;
+template <int TXW, int NTH, int TY, bool SRC, class SM>
+__device__ __forceinline__ void halo_commit(const Halo<TXW, NTH, TY>& hr, SM& sm) {
   using HR = Halo<TXW, NTH, TY>;
   const int tid = threadIdx.x;
 #pragma unroll
@@ -211,16 +218,17 @@ __device__ __forceinline__ void halo_commit(const Halo<TXW, NTH, TY>& hr,
 // Phase D of one tile: each wave takes whole 64-column rows of output nodes;
 // per node it collides the nine populations at their source nodes (SRC:
 // reads them, collided at the source in phase C), streams them (pull: f_k(x)
-// = fo_k(x - c_k)), stores them (WF) and forms h, ux, uy (and p, WF).  Stores
+// = fo_k(x - c_k)), stores them (WF) and forms h, ux, uy (and p, WF, unless
+// the layout shares p's buffer: then the caller stores p first).  Stores
 // address each plane as a row base that is uniform across the wave plus the
 // lane's column, so no per-store 64-bit address arithmetic is needed.  GRAV
 // = false drops the gravity terms (g = 0: they are exactly zero).
-template <int TXW, int NTH, int TY, bool READ_FT, bool SRC, bool SERIES, bool WF, bool GRAV>
+template <int TXW, int NTH, int TY, bool READ_FT, bool SRC, bool SERIES, bool WF, bool GRAV,
+          class SM>
 __device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const TileRun& R, int x0,
-                                        int y0, Smem<TXW, TY, SRC>& sm, SeriesAcc& acc) {
+                                        int y0, SM& sm, SeriesAcc& acc) {
 #pragma clang fp contract(fast)
   constexpr int NW = NTH / 64;
-  using SM = Smem<TXW, TY, SRC>;
   constexpr int HC = SM::HC, PC = SM::PC, UC = SM::UC, UN = SM::UN;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -286,7 +294,7 @@ __device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const T
     store_f(at(io.h_out), s, ntm);
     store_f(at(io.ux_out), ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, ntm);
     store_f(at(io.uy_out), ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, ntm);
-    if constexpr (WF) store_f(at(io.p_out), sm.p[(row + 2) * PC + (col + 2)], R.nts);
+    if constexpr (WF && !SM::AL) store_f(at(io.p_out), sm.p()[(row + 2) * PC + (col + 2)], R.nts);
     if constexpr (SERIES) {
       const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (col + 3)];
       acc.lmax = hv > acc.lmax ? hv : acc.lmax;
@@ -298,16 +306,14 @@ __device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const T
 
 // Phases B-D of one TXW x TY tile whose halo is in LDS: pressure, forces,
 // collision, pull-stream and moments, computed by NTH threads.
-template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool SERIES>
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool SERIES, class SM>
 __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, const TileRun& R,
-                                             int x0, int y0, Smem<TXW, TY, SRC>& sm,
-                                             SeriesAcc& acc) {
+                                             int x0, int y0, SM& sm, SeriesAcc& acc) {
 #pragma clang fp contract(fast)
   static_assert(TXW % 64 == 0 && NTH % 64 == 0, "tiles are whole wavefronts wide");
   constexpr int NT = NTH, NW = NTH / 64;
-  using SM = Smem<TXW, TY, SRC>;
-  constexpr int HC = SM::HC;
-  constexpr int PC = SM::PC, PN = SM::PN;
+  constexpr bool AL = SM::AL;
+  constexpr int HC = SM::HC, PC = SM::PC;
   constexpr int UC = SM::UC, UR = SM::UR, UN = SM::UN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -323,6 +329,7 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
   // column per lane, so each LDS access of a wave is one contiguous run of a
   // row (no bank conflicts where a flat index would wrap into the next row);
   // the PC - TXW columns beyond them follow, a share per wave.
+  float* const P = sm.p();
   auto pnode = [&](int r, int c) {
     const float* hc = &sm.h[(r + 1) * HC + (c + 1)];
     const float hip = hc[-1], him = hc[1], hjp = hc[-HC], hjm = hc[HC];
@@ -331,7 +338,7 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
                     (1.0f / 6.0f) * (((hipjp + himjp) + himjm) + hipjm) - (10.0f / 3.0f) * hc[0];
     float kap = K.kap;
     if (R.use_kappa) kap = io.kappa[wx(x0 - 2 + c) + (int64_t)Lx * wy(y0 - 2 + r)];
-    sm.p[r * PC + c] = disjoining_f(K, hc[0], kap, R.use_kappa) - K.gamma * L;
+    P[r * PC + c] = disjoining_f(K, hc[0], kap, R.use_kappa) - K.gamma * L;
   };
   {
     constexpr int WPR = TXW / 64, PR = SM::PR, XB = PC - TXW, LB = XB * PR;
@@ -346,10 +353,24 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
   }
   __syncthreads();
 
-  // ---- C: forces (and with SRC the collision) on the 1-halo
+  // ---- C: forces (and with SRC the collision) on the 1-halo.  With AL the
+  // (Fx, Fy) pairs stay in registers until every wave has read p, then
+  // overwrite it; p at the wave's phase D nodes waits in registers too, for
+  // stores after the block's last barrier (a barrier waits for the wave's
+  // stores to complete).
+  constexpr int NRW = (TY * (TXW / 64) + NW - 1) / NW;  // phase D rows per wave
+  float pown[NRW];
+  auto keep_p = [&]() {
+#pragma unroll
+    for (int i = 0; i < NRW; ++i) {
+      const int rc = wv + i * NW, row = rc / (TXW / 64), col = lane + 64 * (rc % (TXW / 64));
+      if (rc < TY * (TXW / 64)) pown[i] = P[(row + 2) * PC + (col + 2)];
+    }
+  };
   const float w1 = 1.0f / 9.0f, w5 = 1.0f / 36.0f;
+  float2* const F2 = reinterpret_cast<float2*>(sm.a);
   auto fnode = [&](int r, int c, float z0, float z1) {
-    const float* pc = &sm.p[(r + 1) * PC + (c + 1)];
+    const float* pc = &P[(r + 1) * PC + (c + 1)];
     const float fip = pc[-1], fim = pc[1], fjp = pc[-PC], fjm = pc[PC];
     const float fipjp = pc[-PC - 1], fimjp = pc[-PC + 1], fimjm = pc[PC + 1], fipjm = pc[PC - 1];
     const float dh = sm.h[(r + 2) * HC + (c + 2)];
@@ -408,9 +429,10 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
         if constexpr (READ_FT) f += K.omeg * io.ft[k * FS + gsrc];
         sm.a[k * UN + ui] = f;
       }
-    } else {
-      reinterpret_cast<float2*>(sm.a)[ui] = make_float2(Fx, Fy);
+    } else if constexpr (!AL) {
+      F2[ui] = make_float2(Fx, Fy);
     }
+    return make_float2(Fx, Fy);
   };
   if (R.pairs) {
     // one Philox call per vertical node pair (x, 2m), (x, 2m+1): pair row q
@@ -418,8 +440,9 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
     // rows of pairs go to the waves whole, one column per lane (contiguous
     // LDS runs, as in phase B); the UC - TXW edge columns go to the last wave
     constexpr int NPR = TY / 2 + 2, WPR = TXW / 64, XC = UC - TXW, LC = XC * NPR;
+    constexpr int NIT = (NPR * WPR + NW - 1) / NW;
     static_assert(LC <= 64, "one pass of the last wave covers the edge columns");
-    auto pair = [&](int q, int c) {
+    auto pair = [&](int q, int c, float2* Fp) {
       const int re = 2 * q - 1, ro = 2 * q;
       float z[4] = {0.f, 0.f, 0.f, 0.f};
       if constexpr (THERMAL) {
@@ -431,16 +454,41 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
         box_muller_f32(w[0], w[1], &z[0], &z[1]);
         box_muller_f32(w[2], w[3], &z[2], &z[3]);
       }
-      if (re >= 0) fnode(re, c, z[0], z[1]);
-      if (ro < UR) fnode(ro, c, z[2], z[3]);
+      if (re >= 0) Fp[0] = fnode(re, c, z[0], z[1]);
+      if (ro < UR) Fp[1] = fnode(ro, c, z[2], z[3]);
     };
-    for (int t = wv; t < NPR * WPR; t += NW) {
-      const int q = t / WPR;
-      pair(q, lane + 64 * (t - q * WPR));
+    auto put = [&](int q, int c, const float2* Fp) {
+      const int re = 2 * q - 1, ro = 2 * q;
+      if (re >= 0) F2[re * UC + c] = Fp[0];
+      if (ro < UR) F2[ro * UC + c] = Fp[1];
+    };
+    const bool edge = wv == NW - 1 && lane < LC;
+    if constexpr (AL) {
+      float2 Fq[NIT + 1][2];  // the last entry: the edge columns
+#pragma unroll
+      for (int i = 0; i < NIT; ++i) {
+        const int t = wv + i * NW, q = t / WPR;
+        if (t < NPR * WPR) pair(q, lane + 64 * (t - q * WPR), Fq[i]);
+      }
+      if (edge) pair(lane / XC, TXW + lane % XC, Fq[NIT]);
+      keep_p();
+      __syncthreads();  // every wave has read p
+#pragma unroll
+      for (int i = 0; i < NIT; ++i) {
+        const int t = wv + i * NW, q = t / WPR;
+        if (t < NPR * WPR) put(q, lane + 64 * (t - q * WPR), Fq[i]);
+      }
+      if (edge) put(lane / XC, TXW + lane % XC, Fq[NIT]);
+    } else {
+      float2 Fp[2];  // fnode has stored the pairs
+      for (int t = wv; t < NPR * WPR; t += NW) {
+        const int q = t / WPR;
+        pair(q, lane + 64 * (t - q * WPR), Fp);
+      }
+      if (edge) pair(lane / XC, TXW + lane % XC, Fp);
     }
-    if (wv == NW - 1 && lane < LC) pair(lane / XC, TXW + lane % XC);
   } else {
-    for (int t = tid; t < UN; t += NT) {
+    auto node = [&](int t) {
       const int r = t / UC, c = t - r * UC;
       float z0 = 0.f, z1 = 0.f;
       if constexpr (THERMAL) {
@@ -450,12 +498,38 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
         const int b = (int)(s & 1u);
         box_muller_f32(w[2 * b], w[2 * b + 1], &z0, &z1);
       }
-      fnode(r, c, z0, z1);
+      return fnode(r, c, z0, z1);
+    };
+    if constexpr (AL) {
+      constexpr int NIU = (UN + NT - 1) / NT;
+      float2 Fn[NIU];
+#pragma unroll
+      for (int i = 0; i < NIU; ++i)
+        if (tid + i * NT < UN) Fn[i] = node(tid + i * NT);
+      keep_p();
+      __syncthreads();  // every wave has read p
+#pragma unroll
+      for (int i = 0; i < NIU; ++i)
+        if (tid + i * NT < UN) F2[tid + i * NT] = Fn[i];
+    } else {
+      for (int t = tid; t < UN; t += NT) node(t);  // fnode stores the pairs
     }
   }
   __syncthreads();
 
   __builtin_amdgcn_s_setprio(1);  // the stores ahead of other waves' pressure and forces
+  if constexpr (AL) {
+    if (io.p_out) {
+#pragma unroll
+      for (int i = 0; i < NRW; ++i) {
+        const int rc = wv + i * NW, row = rc / (TXW / 64), cb = 64 * (rc % (TXW / 64));
+        if (rc < TY * (TXW / 64) && y0 + row < Ly && x0 + cb + lane < Lx)
+          store_f((float*)((char*)(io.p_out + ((int64_t)Lx * (y0 + row) + x0 + cb)) +
+                           4u * (uint32_t)lane),
+                  pown[i], R.nts);
+      }
+    }
+  }
   // ---- D: collide, stream (gather) and moments; one copy per store set and
   // gravity, chosen once per tile (uniform), so the loop body has no branches
   if (io.f_out) {
@@ -472,10 +546,9 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
 // One TXW x TY tile of one film: stage the halo, then phases B-D.  The caller
 // synchronises the block between two calls (the tile rewrites the LDS halo
 // first).
-template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool SERIES>
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool SERIES, class SM>
 __device__ __forceinline__ void tile_step(const TileIO& io, const F32K& K, const TileRun& R,
-                                          int x0, int y0, Smem<TXW, TY, SRC>& sm,
-                                          SeriesAcc& acc) {
+                                          int x0, int y0, SM& sm, SeriesAcc& acc) {
   Halo<TXW, NTH, TY> hr;
   __builtin_amdgcn_s_setprio(2);  // issue the tile's loads ahead of other waves' arithmetic
   halo_load<TXW, NTH, TY>(io, R, x0, y0, hr);
@@ -758,10 +831,9 @@ __device__ __forceinline__ void series_flush(const SeriesAcc& acc, const StepSer
   }
 }
 
-template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC>
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, class SM>
 __device__ __forceinline__ void single_film_step(const TileIO& io, const F32K& K, const TileRun& R,
-                                                 int ntx, int ntiles, const StepSeries& ser,
-                                                 Smem<TXW, TY, SRC>& sm,
+                                                 int ntx, int ntiles, const StepSeries& ser, SM& sm,
                                                  unsigned long long* red_k, unsigned int* red_c) {
   constexpr int NW = NTH / 64;
   const int tile = tile_of_block(ntx, ntiles, R.step);
@@ -773,10 +845,11 @@ __device__ __forceinline__ void single_film_step(const TileIO& io, const F32K& K
   if (series) series_flush<NW>(acc, ser, red_k, red_c);
 }
 
-template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC>
+// AL: p and the (Fx, Fy) pairs share one LDS buffer (never with SRC).
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool AL>
 __global__ __launch_bounds__(NTH) void k_step_f32(TileIO io, F32K K, TileRun R, int ntx,
                                                   int ntiles, StepSeries ser) {
-  __shared__ Smem<TXW, TY, SRC> sm;
+  __shared__ Smem<TXW, TY, SRC, AL> sm;
   __shared__ unsigned long long red_k[2 * (NTH / 64)];
   __shared__ unsigned int red_c[NTH / 64];
   single_film_step<TXW, NTH, TY, READ_FT, THERMAL, SRC>(io, K, R, ntx, ntiles, ser, sm, red_k,
@@ -787,14 +860,30 @@ __global__ __launch_bounds__(NTH) void k_step_f32(TileIO io, F32K K, TileRun R, 
 // distributions and pressure are not written (f_out = p_out = null): the
 // identical body with the same runtime checks, so both give the same bits,
 // and a kernel trace tells the two kinds of step apart.
-template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC>
+template <int TXW, int NTH, int TY, bool READ_FT, bool THERMAL, bool SRC, bool AL>
 __global__ __launch_bounds__(NTH) void k_step_f32_unwritten(TileIO io, F32K K, TileRun R,
                                                             int ntx, int ntiles, StepSeries ser) {
-  __shared__ Smem<TXW, TY, SRC> sm;
+  __shared__ Smem<TXW, TY, SRC, AL> sm;
   __shared__ unsigned long long red_k[2 * (NTH / 64)];
   __shared__ unsigned int red_c[NTH / 64];
   single_film_step<TXW, NTH, TY, READ_FT, THERMAL, SRC>(io, K, R, ntx, ntiles, ser, sm, red_k,
                                                         red_c);
+}
+
+// Compute units of the current device, asked once per device (256 if the
+// runtime cannot say).
+int device_cus() {
+  static int cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        c <= 0)
+      c = 256;
+    cus[dev] = c;
+  }
+  return cus[dev];
 }
 
 template <int TXW, int NTH, int TY, bool SRC>
@@ -817,9 +906,19 @@ void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser
   R.wet_thresh = sp.wet_thresh;
   const F32K K = make_f32k(sp, enc);
   const dim3 grid(ntx * nty), block(NTH);
-#define SW_L(KN, RF, TH)                                                                     \
-  hipLaunchKernelGGL((KN<TXW, NTH, TY, RF, TH, SRC>), grid, block, 0, s, t, K, R, ntx, ntx * nty, \
-                     ser)
+  // The shared p / (Fx, Fy) buffer raises a 64 x 16 tile's residency from 5
+  // to 6 workgroups per CU for one more barrier per tile: it pays only where
+  // the film has more tiles than 5 per CU can hold at once.
+  const bool al = !SRC && TY >= 16 && (int64_t)ntx * nty > 5 * (int64_t)device_cus();
+#define SW_L(KN, RF, TH)                                                                        \
+  do {                                                                                          \
+    if (al)                                                                                     \
+      hipLaunchKernelGGL((KN<TXW, NTH, TY, RF, TH, SRC, !SRC>), grid, block, 0, s, t, K, R, ntx, \
+                         ntx * nty, ser);                                                       \
+    else                                                                                        \
+      hipLaunchKernelGGL((KN<TXW, NTH, TY, RF, TH, SRC, false>), grid, block, 0, s, t, K, R,     \
+                         ntx, ntx * nty, ser);                                                  \
+  } while (0)
   if (sp.read_ft) {
     if (sp.thermal) SW_L(k_step_f32, true, true);
     else SW_L(k_step_f32, true, false);
@@ -848,10 +947,12 @@ k_batch_f32(const BatchFilm* __restrict__ films, const int* __restrict__ tstart,
             const BatchHdr* __restrict__ hdr, const unsigned long long* __restrict__ counter,
             int nts) {
   // LDS for the taller tile; a film of 8-row tiles uses its front part
-  static_assert(sizeof(Smem<TX, 8, false>) <= sizeof(Smem<TX, kBatchTileY, false>),
-                "8-row tiles fit the 16-row tile's LDS");
-  __shared__ Smem<TX, kBatchTileY, false> sm;
-  Smem<TX, 8, false>& sm8 = *reinterpret_cast<Smem<TX, 8, false>*>(&sm);
+  // p and the (Fx, Fy) pairs apart: sharing them costs this kernel spills
+  using SM16 = Smem<TX, kBatchTileY, false, false>;
+  using SM8 = Smem<TX, 8, false, false>;
+  static_assert(sizeof(SM8) <= sizeof(SM16), "8-row tiles fit the 16-row tile's LDS");
+  __shared__ SM16 sm;
+  SM8& sm8 = *reinterpret_cast<SM8*>(&sm);
   const int total = hdr->total_tiles, nf = hdr->nfilms;
   const unsigned long long L = *counter;
   // XCD x = b mod 8 owns list indices 8x .. 8x + 7 of every 64 (neighbours

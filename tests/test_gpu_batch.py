@@ -116,6 +116,35 @@ def test_batch_matches_each_film_alone_bitwise(S):
     b.close()
 
 
+def test_large_film_alone_matches_batch(S):
+    """A film with more 64 x 16 tiles than five per CU hold at once (2048 x
+    1024: 2048 tiles) steps alone with p and the (Fx, Fy) pairs sharing one
+    LDS buffer; the batch kernel keeps the two apart.  Both give the same
+    bits: τ = 1 with noise and τ = 0.8, alone with every output written each
+    step and with the final outputs only."""
+    from swalbe import serve
+    T, fi = S.Taumucs, S.film_init
+    specs = [
+        (2048, 1024, T(Tmax=1, kbt=1e-6, γ=0.01), fi(S.INIT_NORMAL, thermal=True, seed=21,
+                                                     base=1.0, amp=1e-3, href=1.0), 40),
+        (2048, 1024, T(Tmax=1, τ=0.8, γ=0.01, θ=1 / 9), fi(S.INIT_CAP, precursor=0.1, href=0.1,
+                                                          cap1=(150.0, 1 / 9, 1024.5, 512.5)),
+         30),
+    ]
+    b = S.Batch(1 << 30, 8)
+    slots = [b.add(prm, Lx, Ly, init, n) for Lx, Ly, prm, init, n in specs]
+    b.step(max(s[4] for s in specs))
+    for slot, (Lx, Ly, prm, init, n) in zip(slots, specs):
+        got = {name: b.get(slot, name) for name in FIELDS}
+        assert np.isfinite(got["height"]).all()
+        for last_only in (False, True):
+            print(f"[large] film {Lx}x{Ly} tau {prm.τ} final_outputs_only {last_only}", flush=True)
+            r = serve.Request(0, Lx, Ly, prm, init, n, fields=FIELDS)
+            ref = serve.solve_alone(r, "fused", final_outputs_only=last_only)
+            assert_same(got, ref, f"large {Lx}x{Ly} tau {prm.τ} last_only {last_only}")
+    b.close()
+
+
 def test_final_outputs_only_batch_matches_alone(S):
     """Films admitted with final_outputs_only write distributions and pressure
     on their last step only: every field at the end is the same bits as the
