@@ -79,9 +79,31 @@ __device__ __forceinline__ float disjoining_f(const F32K& K, float dh, float kap
   return -K.gamma * (kap * powlaw_f(a, K.n, K.m, K.powmode));
 }
 
+// p, a pointer uniform over the wave, as the compiler's scalar registers hold
+// it: an address formed from it plus a 32-bit lane offset is a store's
+// scalar base and offset, with no 64-bit vector add per store (left alone,
+// the compiler adds the lane offset to the plane base first and the row
+// offset to that per store, in 64-bit vector arithmetic).
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ gchar* scalar_ptr(float* p) {
+  const uint64_t a = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return (gchar*)(((uint64_t)hi << 32) | lo);
+}
+
+// Loads and stores of the tile code go through the global address space.
+// The batch kernel takes its films' pointers from a table in memory, and
+// through plain pointers the compiler would address them with flat
+// instructions, which also count against the LDS counter: every wait for an
+// LDS read would then wait for the outstanding stores too.
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ float gld(const float* p) { return *(const gfloat*)p; }
+
 __device__ __forceinline__ void store_f(float* p, float v, int nts) {
-  if (nts) __builtin_nontemporal_store(v, p);
-  else *p = v;
+  gfloat* g = (gfloat*)p;
+  if (nts) __builtin_nontemporal_store(v, g);
+  else *g = v;
 }
 
 // LDS of one tile: h on the 3-halo, p on the 2-halo, (ux, uy) and the phase C
@@ -161,7 +183,7 @@ __device__ __forceinline__ void halo_load(const TileIO& io, const TileRun& R, in
       const int e = tid + i * NTH;
       if (e < HN) {
         const int r = e / HC, c = e - r * HC;
-        hr.h[i] = *(const float*)(hb + (uint32_t)(4 * (c + Lx * r)));
+        hr.h[i] = gld((const float*)(hb + (uint32_t)(4 * (c + Lx * r))));
       }
     }
 #pragma unroll
@@ -170,8 +192,8 @@ __device__ __forceinline__ void halo_load(const TileIO& io, const TileRun& R, in
       if (e < UN) {
         const int r = e / UC, c = e - r * UC;
         const uint32_t o = (uint32_t)(4 * (c + Lx * r));
-        hr.u[i] = *(const float*)(ub + o);
-        hr.v[i] = *(const float*)(vb + o);
+        hr.u[i] = gld((const float*)(ub + o));
+        hr.v[i] = gld((const float*)(vb + o));
       }
     }
     return;
@@ -183,7 +205,7 @@ __device__ __forceinline__ void halo_load(const TileIO& io, const TileRun& R, in
     const int e = tid + i * NTH;
     if (e < HN) {
       const int r = e / HC, c = e - r * HC;
-      hr.h[i] = io.h_in[wx(x0 - 3 + c) + (int64_t)Lx * wy(y0 - 3 + r)];
+      hr.h[i] = gld(&io.h_in[wx(x0 - 3 + c) + (int64_t)Lx * wy(y0 - 3 + r)]);
     }
   }
 #pragma unroll
@@ -192,8 +214,8 @@ __device__ __forceinline__ void halo_load(const TileIO& io, const TileRun& R, in
     if (e < UN) {
       const int r = e / UC, c = e - r * UC;
       const int64_t g = wx(x0 - 1 + c) + (int64_t)Lx * wy(y0 - 1 + r);
-      hr.u[i] = io.ux_in[g];
-      hr.v[i] = io.uy_in[g];
+      hr.u[i] = gld(&io.ux_in[g]);
+      hr.v[i] = gld(&io.uy_in[g]);
     }
   }
 }
@@ -245,8 +267,11 @@ __device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const T
     // the output row's first node: uniform, so plane + rowo is a scalar base
     // and the lane's 32-bit byte offset the only per-lane part of an address
     const int64_t rowo = (int64_t)Lx * gyo + x0 + cb;
-    const uint32_t lo = 4u * (uint32_t)lane;
-    auto at = [&](float* plane) { return (float*)((char*)(plane + rowo) + lo); };
+    // opaque per row, so the compiler keeps it a 32-bit offset beside the
+    // stores (hoisted, it becomes a 64-bit vector operand of each address)
+    uint32_t lo = 4u * (uint32_t)lane;
+    asm volatile("" : "+v"(lo));
+    auto at = [&](float* plane) { return (float*)(scalar_ptr(plane + rowo) + lo); };
     float fo[kQ];
 #pragma unroll
     for (int k = 0; k < kQ; ++k) {
@@ -281,7 +306,7 @@ __device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const T
         }
         if constexpr (READ_FT) {
           const int sx = wrap1(x0 + col - cx(k), Lx), sy = wrap1(gyo - cy(k), Ly);
-          f += K.omeg * io.ft[k * FS + sx + (int64_t)Lx * sy];
+          f += K.omeg * gld(&io.ft[k * FS + sx + (int64_t)Lx * sy]);
         }
       }
       fo[k] = f;
@@ -337,7 +362,7 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
     const float L = (2.0f / 3.0f) * (((hjp + hip) + him) + hjm) +
                     (1.0f / 6.0f) * (((hipjp + himjp) + himjm) + hipjm) - (10.0f / 3.0f) * hc[0];
     float kap = K.kap;
-    if (R.use_kappa) kap = io.kappa[wx(x0 - 2 + c) + (int64_t)Lx * wy(y0 - 2 + r)];
+    if (R.use_kappa) kap = gld(&io.kappa[wx(x0 - 2 + c) + (int64_t)Lx * wy(y0 - 2 + r)]);
     P[r * PC + c] = disjoining_f(K, hc[0], kap, R.use_kappa) - K.gamma * L;
   };
   {
@@ -426,7 +451,7 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
 #pragma unroll
       for (int k = 0; k < kQ; ++k) {
         float f = READ_FT ? K.itau * fe[k] + S[k] : fe[k] + S[k];
-        if constexpr (READ_FT) f += K.omeg * io.ft[k * FS + gsrc];
+        if constexpr (READ_FT) f += K.omeg * gld(&io.ft[k * FS + gsrc]);
         sm.a[k * UN + ui] = f;
       }
     } else if constexpr (!AL) {
@@ -523,10 +548,12 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
 #pragma unroll
       for (int i = 0; i < NRW; ++i) {
         const int rc = wv + i * NW, row = rc / (TXW / 64), cb = 64 * (rc % (TXW / 64));
-        if (rc < TY * (TXW / 64) && y0 + row < Ly && x0 + cb + lane < Lx)
-          store_f((float*)((char*)(io.p_out + ((int64_t)Lx * (y0 + row) + x0 + cb)) +
-                           4u * (uint32_t)lane),
+        if (rc < TY * (TXW / 64) && y0 + row < Ly && x0 + cb + lane < Lx) {
+          uint32_t lo = 4u * (uint32_t)lane;  // a 32-bit store offset, as in phase D
+          asm volatile("" : "+v"(lo));
+          store_f((float*)(scalar_ptr(io.p_out + ((int64_t)Lx * (y0 + row) + x0 + cb)) + lo),
                   pown[i], R.nts);
+        }
       }
     }
   }
