@@ -575,8 +575,9 @@ def main():
                     help="1D resident kernel: threads per film (0 = by L, 256, 512, 1024)")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0)
-    ap.add_argument("--nontemporal", type=int, default=2,
-                    help="FP32 stores: 0 cached, 1 all streaming, 2 f and p streaming")
+    ap.add_argument("--nontemporal", type=int, default=-1,
+                    help="fused-step stores: 0 cached, 1 all streaming, 2 f and p streaming; "
+                         "default the faster for the film: 0 for FP32 films, 2 for FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="default workload: skip the config-5 serving measurement")
@@ -631,7 +632,7 @@ def main():
     prm = param.cparams()
     o = _lib.StepOpts()
     o.thermal, o.seed, o.tile_y, o.variant = int(thermal), 7, a.tile_y, a.variant
-    o.nontemporal = a.nontemporal
+    o.nontemporal = a.nontemporal if a.nontemporal >= 0 else (0 if dt == "f32" else 2)
     ms = C.c_double()
     L = _lib.lib()
     # the live ceilings, then an untimed device prewarm: the part reaches its
@@ -716,7 +717,7 @@ def main():
                                + f"{Lx}x{Ly} periodic, theta=1/9, n=9, m=3, gamma=0.01, tau=1",
                    "Lx": Lx, "Ly": Ly, "parallelism": "single GPU",
                    "tile_y": a.tile_y or (16 if dt == "f32" else 8),
-                   "variant": a.variant, "nontemporal": a.nontemporal,
+                   "variant": a.variant, "nontemporal": o.nontemporal,
                    "plane_pad": int(os.environ.get("SWALBE_PLANE_PAD", "0")), "finite": ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -160,7 +160,7 @@ typedef struct swalbe_step_opts {
     int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = LDS tiles, collide at the
                               source node, 2 = LDS tiles, collide at the destination node,
                               3 = LDS-free strips (one wavefront per 58-column strip) */
-    int32_t nontemporal;   /* FP32 kernel stores (results are the same bits either way):
+    int32_t nontemporal;   /* fused-step stores, FP32 and FP64 (the same bits either way):
                               0 = cached, 1 = every output streams past the caches
                               (non-temporal), 2 = the distributions and the pressure
                               stream while h, ux, uy (the next step's input) stay cached */
@@ -264,7 +264,7 @@ int swalbe_batch_step(swalbe_batch *b, int64_t nsteps);
 int swalbe_batch_step_timed(swalbe_batch *b, int64_t nsteps, double *ms);
 /* replay runs of steps_per_graph steps as one captured HIP graph (0 = off) */
 int swalbe_batch_set_graph(swalbe_batch *b, int32_t steps_per_graph);
-/* stores of the batch's films: 0, 1 or 2 as swalbe_step_opts.nontemporal */
+/* stores of the batch's films: 0, 1 or 2 as swalbe_step_opts.nontemporal (default 2) */
 int swalbe_batch_set_nontemporal(swalbe_batch *b, int32_t on);
 /* Autotuned tile heights: every film runs in tiles of 64 columns by 8 or 16
  * rows (16 by default).  With autotune on, a film admitted afterwards takes

@@ -872,7 +872,7 @@ void enqueue_step(swalbe_state* st, const StepParams& sp, StepSeries ser, uint64
     io.p_out = nullptr;
   }
   if constexpr (sizeof(T) == 8)
-    launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.fstride, F.enc, tile_y, st->stream);
+    launch_step<T>(io, sp, ser, t, F.Lx, F.Ly, F.fstride, F.enc, tile_y, nts, st->stream);
   else
     launch_step_f32(io, sp, ser, t, F.Lx, F.Ly, F.fstride, F.enc, tile_y, variant, nts,
                     st->stream);

@@ -101,7 +101,18 @@ __device__ __forceinline__ T force_term(int k, T Fx, T Fy) {
   }
 }
 
-template <typename T, int TY, bool READ_FT, bool THERMAL>
+// A store that streams past the caches (non-temporal) when NT.  The choice is
+// a template argument: behind a run-time branch the compiler merges the two
+// forms into one plain store.
+template <bool NT, typename T>
+__device__ __forceinline__ void store_t(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// NTS: 0 every output cached, 1 every output streaming, 2 the distributions
+// and p streaming while h, ux, uy (the next step's input) stay cached.
+template <typename T, int TY, bool READ_FT, bool THERMAL, int NTS>
 __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSeries ser,
                                              uint64_t step, int Lx, int Ly, int ntx, int ntiles,
                                              int small, int64_t FS, Enc enc) {
@@ -325,16 +336,16 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
       }
       if (k > 0) f = f + force_term<T>(k, sh_fx[su], sh_fy[su]);
       fo[k] = f;
-      if (io.f_out) io.f_out[k * FS + xo] = f;
+      if (io.f_out) store_t<NTS != 0>(&io.f_out[k * FS + xo], f);
     }
     T s = fo[0];
 #pragma unroll
     for (int k = 1; k < kQ; ++k) s = s + fo[k];
     const T hnew = DEV ? href + s : s;
-    io.h_out[xo] = s;
-    io.ux_out[xo] = (((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7]) + fo[8]) / hnew;
-    io.uy_out[xo] = (((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7]) - fo[8]) / hnew;
-    if (io.p_out) io.p_out[xo] = sh_p[(row + 2) * PC + (tx + 2)];
+    store_t<NTS == 1>(&io.h_out[xo], s);
+    store_t<NTS == 1>(&io.ux_out[xo], (((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7]) + fo[8]) / hnew);
+    store_t<NTS == 1>(&io.uy_out[xo], (((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7]) - fo[8]) / hnew);
+    if (io.p_out) store_t<NTS != 0>(&io.p_out[xo], sh_p[(row + 2) * PC + (tx + 2)]);
     if (ser.dh_keys || ser.wet) {
       const T hc = sh_h[(row + 3) * HC + (tx + 3)];
       const double hv = DEV ? (double)href + (double)hc : (double)hc;
@@ -377,26 +388,29 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
 
 template <typename T, int TY>
 void launch_ty(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step, int Lx,
-               int Ly, int64_t FS, Enc enc, hipStream_t s) {
+               int Ly, int64_t FS, Enc enc, int nts, hipStream_t s) {
   const int ntx = (Lx + TX - 1) / TX, nty = (Ly + TY - 1) / TY;
   const int ntiles = ntx * nty;
   const int small = (Lx < TX + 6 || Ly < TY + 6) ? 1 : 0;
   dim3 grid(ntiles), block(NT);
+#define SW_K(RF, TH, M)                                                                      \
+  hipLaunchKernelGGL((k_step<T, TY, RF, TH, M>), grid, block, 0, s, io, sp, ser, step, Lx, Ly, \
+                     ntx, ntiles, small, FS, enc)
+#define SW_M(RF, TH)          \
+  do {                        \
+    if (nts == 2) SW_K(RF, TH, 2); \
+    else if (nts == 1) SW_K(RF, TH, 1); \
+    else SW_K(RF, TH, 0);     \
+  } while (0)
   if (sp.read_ft) {
-    if (sp.thermal)
-      hipLaunchKernelGGL((k_step<T, TY, true, true>), grid, block, 0, s, io, sp, ser, step, Lx, Ly,
-                         ntx, ntiles, small, FS, enc);
-    else
-      hipLaunchKernelGGL((k_step<T, TY, true, false>), grid, block, 0, s, io, sp, ser, step, Lx,
-                         Ly, ntx, ntiles, small, FS, enc);
+    if (sp.thermal) SW_M(true, true);
+    else SW_M(true, false);
   } else {
-    if (sp.thermal)
-      hipLaunchKernelGGL((k_step<T, TY, false, true>), grid, block, 0, s, io, sp, ser, step, Lx,
-                         Ly, ntx, ntiles, small, FS, enc);
-    else
-      hipLaunchKernelGGL((k_step<T, TY, false, false>), grid, block, 0, s, io, sp, ser, step, Lx,
-                         Ly, ntx, ntiles, small, FS, enc);
+    if (sp.thermal) SW_M(false, true);
+    else SW_M(false, false);
   }
+#undef SW_M
+#undef SW_K
 }
 
 // ---------------------------------------------------------------- reductions
@@ -468,10 +482,10 @@ __global__ void k_reduce_finish(const double* __restrict__ partials, int nparts,
 
 template <typename T>
 void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step,
-                 int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, hipStream_t s) {
+                 int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, int nts, hipStream_t s) {
   switch (tile_y) {
-    case 8: launch_ty<T, 8>(io, sp, ser, step, Lx, Ly, fstride, enc, s); break;
-    default: launch_ty<T, 16>(io, sp, ser, step, Lx, Ly, fstride, enc, s); break;
+    case 8: launch_ty<T, 8>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s); break;
+    default: launch_ty<T, 16>(io, sp, ser, step, Lx, Ly, fstride, enc, nts, s); break;
   }
 }
 
@@ -487,7 +501,7 @@ void launch_reduce_finish(const double* partials, int nparts, double* out, hipSt
 }
 
 template void launch_step<double>(const StepIO<double>&, const StepParams&, StepSeries, uint64_t,
-                                  int, int, int64_t, Enc, int, hipStream_t);
+                                  int, int, int64_t, Enc, int, int, hipStream_t);
 template void launch_reduce<double>(const double*, int64_t, double, double, double*, int,
                                     hipStream_t);
 template void launch_reduce<float>(const float*, int64_t, double, double, double*, int,

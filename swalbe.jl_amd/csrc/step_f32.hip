@@ -19,6 +19,8 @@
 // One device function, tile_step, computes a tile; the single-film and the
 // batched kernels both call it, so a film gives the same bits alone and in
 // a batch.
+#include <type_traits>
+
 #include "swalbe_batch.h"
 
 namespace swalbe {
@@ -100,10 +102,24 @@ __device__ __forceinline__ gchar* scalar_ptr(float* p) {
 typedef __attribute__((address_space(1))) float gfloat;
 __device__ __forceinline__ float gld(const float* p) { return *(const gfloat*)p; }
 
-__device__ __forceinline__ void store_f(float* p, float v, int nts) {
+// A store of one output value, non-temporal (streaming past the caches) when
+// NT.  The choice is a template argument: behind a run-time branch the
+// compiler merges the two forms into one plain store.
+template <bool NT>
+__device__ __forceinline__ void store_f(float* p, float v) {
   gfloat* g = (gfloat*)p;
-  if (nts) __builtin_nontemporal_store(v, g);
+  if constexpr (NT) __builtin_nontemporal_store(v, g);
   else *g = v;
+}
+
+// f(std::integral_constant<int, m>) for the run-time store mode nts: 0 every
+// output cached, 1 every output streaming, 2 the distributions and p
+// streaming while h, ux, uy (the next step's input) stay cached.
+template <class Fn>
+__device__ __forceinline__ void with_store_mode(int nts, Fn&& f) {
+  if (nts == 2) f(std::integral_constant<int, 2>{});
+  else if (nts == 1) f(std::integral_constant<int, 1>{});
+  else f(std::integral_constant<int, 0>{});
 }
 
 // LDS of one tile: h on the 3-halo, p on the 2-halo, (ux, uy) and the phase C
@@ -246,7 +262,7 @@ __device__ __forceinline__ void halo_commit(const Halo<TXW, NTH, TY>& hr, SM& sm
 // lane's column, so no per-store 64-bit address arithmetic is needed.  GRAV
 // = false drops the gravity terms (g = 0: they are exactly zero).
 template <int TXW, int NTH, int TY, bool READ_FT, bool SRC, bool SERIES, bool WF, bool GRAV,
-          class SM>
+          int NTS, class SM>
 __device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const TileRun& R, int x0,
                                         int y0, SM& sm, SeriesAcc& acc) {
 #pragma clang fp contract(fast)
@@ -310,16 +326,16 @@ __device__ __forceinline__ void phase_d(const TileIO& io, const F32K& K, const T
         }
       }
       fo[k] = f;
-      if constexpr (WF) store_f(at(io.f_out + k * FS), f, R.nts);
+      if constexpr (WF) store_f<NTS != 0>(at(io.f_out + k * FS), f);
     }
     const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) + fo[8];
     const float ih = __builtin_amdgcn_rcpf(K.href + s);
-    // nts 2: only f and p stream past the caches (h, u are the next step's input)
-    const int ntm = R.nts == 1 ? 1 : 0;
-    store_f(at(io.h_out), s, ntm);
-    store_f(at(io.ux_out), ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, ntm);
-    store_f(at(io.uy_out), ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, ntm);
-    if constexpr (WF && !SM::AL) store_f(at(io.p_out), sm.p()[(row + 2) * PC + (col + 2)], R.nts);
+    // mode 2: only f and p stream past the caches (h, u are the next step's input)
+    store_f<NTS == 1>(at(io.h_out), s);
+    store_f<NTS == 1>(at(io.ux_out), ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih);
+    store_f<NTS == 1>(at(io.uy_out), ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih);
+    if constexpr (WF && !SM::AL)
+      store_f<NTS != 0>(at(io.p_out), sm.p()[(row + 2) * PC + (col + 2)]);
     if constexpr (SERIES) {
       const double hv = (double)K.href + (double)sm.h[(row + 3) * HC + (col + 3)];
       acc.lmax = hv > acc.lmax ? hv : acc.lmax;
@@ -543,29 +559,38 @@ __device__ __forceinline__ void tile_compute(const TileIO& io, const F32K& K, co
   __syncthreads();
 
   __builtin_amdgcn_s_setprio(1);  // the stores ahead of other waves' pressure and forces
-  if constexpr (AL) {
-    if (io.p_out) {
+  // ---- D: collide, stream (gather) and moments; one copy per store mode,
+  // store set and gravity, chosen once per tile (uniform), so the loop body
+  // has no branches
+  with_store_mode(R.nts, [&](auto mode) {
+    constexpr int NTS = decltype(mode)::value;
+    if constexpr (AL) {
+      if (io.p_out) {
 #pragma unroll
-      for (int i = 0; i < NRW; ++i) {
-        const int rc = wv + i * NW, row = rc / (TXW / 64), cb = 64 * (rc % (TXW / 64));
-        if (rc < TY * (TXW / 64) && y0 + row < Ly && x0 + cb + lane < Lx) {
-          uint32_t lo = 4u * (uint32_t)lane;  // a 32-bit store offset, as in phase D
-          asm volatile("" : "+v"(lo));
-          store_f((float*)(scalar_ptr(io.p_out + ((int64_t)Lx * (y0 + row) + x0 + cb)) + lo),
-                  pown[i], R.nts);
+        for (int i = 0; i < NRW; ++i) {
+          const int rc = wv + i * NW, row = rc / (TXW / 64), cb = 64 * (rc % (TXW / 64));
+          if (rc < TY * (TXW / 64) && y0 + row < Ly && x0 + cb + lane < Lx) {
+            uint32_t lo = 4u * (uint32_t)lane;  // a 32-bit store offset, as in phase D
+            asm volatile("" : "+v"(lo));
+            store_f<NTS != 0>(
+                (float*)(scalar_ptr(io.p_out + ((int64_t)Lx * (y0 + row) + x0 + cb)) + lo),
+                pown[i]);
+          }
         }
       }
     }
-  }
-  // ---- D: collide, stream (gather) and moments; one copy per store set and
-  // gravity, chosen once per tile (uniform), so the loop body has no branches
-  if (io.f_out) {
-    if (K.g0 != 0.0f) phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, true, true>(io, K, R, x0, y0, sm, acc);
-    else phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, true, false>(io, K, R, x0, y0, sm, acc);
-  } else {
-    if (K.g0 != 0.0f) phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, false, true>(io, K, R, x0, y0, sm, acc);
-    else phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, false, false>(io, K, R, x0, y0, sm, acc);
-  }
+    if (io.f_out) {
+      if (K.g0 != 0.0f)
+        phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, true, true, NTS>(io, K, R, x0, y0, sm, acc);
+      else
+        phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, true, false, NTS>(io, K, R, x0, y0, sm, acc);
+    } else {
+      if (K.g0 != 0.0f)
+        phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, false, true, NTS>(io, K, R, x0, y0, sm, acc);
+      else
+        phase_d<TXW, NTH, TY, READ_FT, SRC, SERIES, false, false, NTS>(io, K, R, x0, y0, sm, acc);
+    }
+  });
   __builtin_amdgcn_s_setprio(0);
 }
 
@@ -612,7 +637,7 @@ struct Row3 {
 __device__ __forceinline__ Row3 row3(float v) { return Row3{v, at_xm1(v), at_xp1(v)}; }
 
 // Rows [y0, y0 + S) of the strip whose first output column is x0.
-template <bool READ_FT, bool THERMAL, bool SERIES>
+template <bool READ_FT, bool THERMAL, bool SERIES, int NTS>
 __device__ __forceinline__ void strip_run(const TileIO& io, const F32K& K, const TileRun& R,
                                           int x0, int y0, int S, SeriesAcc& acc) {
 #pragma clang fp contract(fast)
@@ -755,16 +780,16 @@ __device__ __forceinline__ void strip_run(const TileIO& io, const F32K& K, const
         const int64_t xo = (int64_t)Lx * j + xc;
         if (io.f_out) {
 #pragma unroll
-          for (int k = 0; k < kQ; ++k) store_f(&io.f_out[k * FS + xo], fo[k], R.nts);
+          for (int k = 0; k < kQ; ++k) store_f<NTS != 0>(&io.f_out[k * FS + xo], fo[k]);
         }
         const float s = (((((((fo[0] + fo[1]) + fo[2]) + fo[3]) + fo[4]) + fo[5]) + fo[6]) + fo[7]) +
                         fo[8];
         const float ih = __builtin_amdgcn_rcpf(K.href + s);
-        const int ntm = R.nts == 1 ? 1 : 0;  // nts 2: h, u stay cacheable
-        store_f(&io.h_out[xo], s, ntm);
-        store_f(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih, ntm);
-        store_f(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih, ntm);
-        if (io.p_out) store_f(&io.p_out[xo], p0.v, R.nts);
+        // mode 2: h, u stay cacheable
+        store_f<NTS == 1>(&io.h_out[xo], s);
+        store_f<NTS == 1>(&io.ux_out[xo], ((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7] + fo[8]) * ih);
+        store_f<NTS == 1>(&io.uy_out[xo], ((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7] - fo[8]) * ih);
+        if (io.p_out) store_f<NTS != 0>(&io.p_out[xo], p0.v);
       }
     }
   };
@@ -803,8 +828,11 @@ __global__ __launch_bounds__(64) void k_strip_f32(TileIO io, F32K K, TileRun R, 
   const int S = min(seg, R.Ly - y0);
   SeriesAcc acc{-__builtin_inf(), __builtin_inf(), 0u};
   const bool series = ser.dh_keys || ser.wet;
-  if (series) strip_run<READ_FT, THERMAL, true>(io, K, R, st * kStripOut, y0, S, acc);
-  else strip_run<READ_FT, THERMAL, false>(io, K, R, st * kStripOut, y0, S, acc);
+  with_store_mode(R.nts, [&](auto mode) {
+    constexpr int NTS = decltype(mode)::value;
+    if (series) strip_run<READ_FT, THERMAL, true, NTS>(io, K, R, st * kStripOut, y0, S, acc);
+    else strip_run<READ_FT, THERMAL, false, NTS>(io, K, R, st * kStripOut, y0, S, acc);
+  });
   if (!series) return;
   unsigned long long kmax = dkey(acc.lmax), kmin = dkey(acc.lmin);
   unsigned int lcnt = acc.cnt;
@@ -1145,11 +1173,17 @@ __global__ __launch_bounds__(64) void k_batch_strip_f32(const BatchFilm* __restr
     const int S = min(kStripSeg, F.Ly - y0);
     const F32K K = F.K;
     if (F.read_ft) {
-      if (F.thermal) strip_run<true, true, false>(io, K, R, st * kStripOut, y0, S, acc);
-      else strip_run<true, false, false>(io, K, R, st * kStripOut, y0, S, acc);
+      with_store_mode(nts, [&](auto mode) {
+        constexpr int NTS = decltype(mode)::value;
+        if (F.thermal) strip_run<true, true, false, NTS>(io, K, R, st * kStripOut, y0, S, acc);
+        else strip_run<true, false, false, NTS>(io, K, R, st * kStripOut, y0, S, acc);
+      });
     } else {
-      if (F.thermal) strip_run<false, true, false>(io, K, R, st * kStripOut, y0, S, acc);
-      else strip_run<false, false, false>(io, K, R, st * kStripOut, y0, S, acc);
+      with_store_mode(nts, [&](auto mode) {
+        constexpr int NTS = decltype(mode)::value;
+        if (F.thermal) strip_run<false, true, false, NTS>(io, K, R, st * kStripOut, y0, S, acc);
+        else strip_run<false, false, false, NTS>(io, K, R, st * kStripOut, y0, S, acc);
+      });
     }
   }
 }

@@ -183,7 +183,7 @@ struct StepIO {
 
 template <typename T>
 void launch_step(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64_t step,
-                 int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, hipStream_t s);
+                 int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, int nts, hipStream_t s);
 // FP32 films: variant 0/1 collide at the source node, 2 at the destination.
 void launch_step_f32(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
                      int Lx, int Ly, int64_t fstride, Enc enc, int tile_y, int variant, int nts,

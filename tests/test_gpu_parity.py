@@ -475,9 +475,11 @@ def test_final_outputs_only_gives_the_same_state(S, T, variant, tau):
     assert sa.keys() == sb.keys() and all(np.array_equal(sa[k], sb[k]) for k in sa)
 
 
-@pytest.mark.parametrize("variant", [0, 3], ids=["tiles", "strips"])
-def test_nontemporal_stores_give_the_same_bits(S, variant):
-    """nontemporal 0 / 1 / 2 change where stores are cached, never a value."""
+@pytest.mark.parametrize("variant,T", [(0, np.float32), (3, np.float32), (0, np.float64)],
+                         ids=["tiles", "strips", "fp64"])
+def test_nontemporal_stores_give_the_same_bits(S, variant, T):
+    """nontemporal 0 / 1 / 2 change where stores are cached, never a value
+    (the FP32 kernels and the FP64 step)."""
     import ctypes as C
     from swalbe import _lib
     Lx, Ly, n = 200, 130, 25
@@ -485,7 +487,7 @@ def test_nontemporal_stores_give_the_same_bits(S, variant):
     got = []
     for mode in (0, 1, 2):
         sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=n, kbt=1e-6))
-        st = S.Sys(sysc, "GPU", T=np.float32)
+        st = S.Sys(sysc, "GPU", T=T)
         st.height = h0
         S.equilibrium(st, sysc)
         o = _lib.StepOpts()
