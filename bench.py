@@ -113,13 +113,15 @@ def cpu_baseline_fused(seconds_target: float = 8.0):
                                       f"{dt:.1f} s, three fused OpenMP sweeps per step, {cpu_model()}"}
 
 
-def batch_run(steps, warmup, graph_steps=100, autotune=True, final_outputs_only=False):
+def batch_run(steps, warmup, graph_steps=100, autotune=True, final_outputs_only=False,
+              nontemporal=-1):
     """256 independent droplet-coalescence films (Lx in {128..1024} step 64,
     Ly = Lx/2, two tangent caps, per-film θ, γ, radius) stepped together by
     the batched FP32 kernel; `steps` timed batch steps after `warmup`, run as
     captured graphs.  Each film's tile height (8 or 16 rows) is the one the
     autotuner measured fastest for its shape (the tuning runs at admission,
-    before the timed steps, and is reported)."""
+    before the timed steps, and is reported).  nontemporal >= 0 sets the
+    films' store mode (otherwise the batch's default, 2)."""
     import numpy as np
     import swalbe as S
     from swalbe import serve
@@ -127,6 +129,8 @@ def batch_run(steps, warmup, graph_steps=100, autotune=True, final_outputs_only=
     cap = sum(S.film_bytes(r.Lx, r.Ly) for r in reqs)
     b = S.Batch(cap, 256, graph_steps=graph_steps, autotune=autotune,
                 final_outputs_only=final_outputs_only)
+    if nontemporal >= 0:
+        b.set_nontemporal(nontemporal)
     t0 = time.perf_counter()
     slots = [b.add(r.param, r.Lx, r.Ly, r.init, r.nsteps) for r in reqs]
     b.sync()
@@ -146,12 +150,14 @@ def batch_run(steps, warmup, graph_steps=100, autotune=True, final_outputs_only=
     ok = all(bool(np.isfinite(b.get(s, "height")).all()) for s in (0, 127, 255))
     b.close()
     return {"glups": glups, "ms": ms, "wall": wall, "nodes": N, "films": st["films"],
-            "cap": cap, "finite": ok, "rows": rows, "shapes": shapes, "admit_s": admit_s}
+            "cap": cap, "finite": ok, "rows": rows, "shapes": shapes, "admit_s": admit_s,
+            "nontemporal": nontemporal if nontemporal >= 0 else 2}
 
 
 def bench_batch(a):
     """Config 4 on its own line (coalesce256_batch): see batch_run()."""
-    r = batch_run(a.steps, a.warmup, a.graph_steps, not a.no_autotune)
+    r = batch_run(a.steps, a.warmup, a.graph_steps, not a.no_autotune,
+                  nontemporal=a.nontemporal)
     bpl = bytes_per_lup(4, True)
     achieved = bpl * r["glups"]
     line = {
@@ -163,6 +169,7 @@ def bench_batch(a):
         "data": "synthetic (seeded two-cap films)",
         "config": {"workload": BATCH_DESC, "films": r["films"], "nodes": r["nodes"],
                    "graph_steps": a.graph_steps, "arena_bytes": r["cap"],
+                   "nontemporal": r["nontemporal"],
                    "parallelism": "single GPU", "finite": r["finite"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -577,7 +584,8 @@ def main():
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--nontemporal", type=int, default=-1,
                     help="fused-step stores: 0 cached, 1 all streaming, 2 f and p streaming; "
-                         "default the faster for the film: 0 for FP32 films, 2 for FP64")
+                         "default the faster for the path: 0 for FP32 films, 2 for FP64 films "
+                         "and batches")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true",
                     help="default workload: skip the config-5 serving measurement")
