@@ -641,18 +641,13 @@ def main():
     prm = param.cparams()
     o = _lib.StepOpts()
     o.thermal, o.seed, o.tile_y, o.variant = int(thermal), 7, a.tile_y, a.variant
-    # store modes (the same bits in each), by default the faster measured for
-    # the pass: FP32 films whose inputs exceed the eight 4 MB L2s stream every
-    # output when every step's outputs are written (mode 1) and cache them when
-    # only the last step's are (0); smaller FP32 films the other way round;
-    # FP64 films stream f and p (2)
-    big = Lx * Ly * 3 * (4 if dt == "f32" else 8) > (32 << 20)
+    # store modes (the same bits in each): by default the faster measured for
+    # the film and the pass (swalbe.store_mode)
     if a.nontemporal >= 0:
         nt_full = nt_final = a.nontemporal
-    elif dt == "f32":
-        nt_full, nt_final = (1, 0) if big else (0, 1)
     else:
-        nt_full = nt_final = 2
+        nt_full = S.store_mode(Lx, Ly, dt == "f64", False)
+        nt_final = S.store_mode(Lx, Ly, dt == "f64", True)
     o.nontemporal = nt_full
     ms = C.c_double()
     L = _lib.lib()

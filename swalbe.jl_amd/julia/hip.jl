@@ -58,11 +58,23 @@ HipStepOpts(; use_theta_field = 0, thermal = 0, body = 0, materialize = 1, seed 
          dh_series = C_NULL, wet_series = C_NULL, mass_series = C_NULL,
          maxh_series = C_NULL, area_series = C_NULL, energy_series = C_NULL,
          fluid_series = C_NULL, surf_thresh = 0.055, surf_theta_field = 0,
-         final_outputs_only = 0) =
+         final_outputs_only = 0, nontemporal = -1) =
     HipStepOpts(use_theta_field, thermal, body, materialize, seed, t0, body_ax, body_ay, body_fac,
-             wet_thresh, dh_series, wet_series, mass_series, 0, 0, 0, 0, 0, C_NULL,
+             wet_thresh, dh_series, wet_series, mass_series, 0, 0, nontemporal, 0, 0, C_NULL,
              maxh_series, area_series, energy_series, fluid_series, surf_thresh,
              surf_theta_field, final_outputs_only)
+
+# The store mode (nontemporal; every mode gives the same bits) measured fastest
+# for a film on MI355X, as swalbe.store_mode of the Python mirror: FP64 films 2;
+# FP32 films with more than 32 MB of inputs 1 when every step writes its
+# outputs, 0 when only the last step does; smaller FP32 films the other way.
+function store_mode(Lx, Ly, ::Type{T}, final_outputs_only) where {T}
+    T === Float64 && return Int32(2)
+    last = final_outputs_only != 0
+    return Int32(Lx * Ly * 12 > (32 << 20) ? (last ? 0 : 1) : (last ? 1 : 0))
+end
+with_store_mode(o::HipStepOpts, mode) =
+    HipStepOpts((f === :nontemporal ? Int32(mode) : getfield(o, f) for f in fieldnames(HipStepOpts))...)
 
 function hip_check(rc::Cint, what)
     rc == HIP_OK && return nothing
@@ -259,7 +271,8 @@ function inclination!(α::Vector, s::HipState; t = 1000, tstart = 0, tsmooth = 1
 end
 
 # ---- time_loop(sys, state[, θ | Δh | f, measure]) (src/simulate.jl:6-96)
-function fused!(sys, s::HipState, o::HipStepOpts; theta = sys.param.θ)
+function fused!(sys, s::HipState{T}, o::HipStepOpts; theta = sys.param.θ) where {T}
+    o.nontemporal < 0 && (o = with_store_mode(o, store_mode(sys.Lx, sys.Ly, T, o.final_outputs_only)))
     p = HipParams(sys.param)
     p = HipParams(p.Tmax, p.tdump, p.tau, p.cs, p.mu, p.delta, p.kbt, p.gamma, p.n, p.m, p.hmin,
                 p.hcrit, theta, p.g)

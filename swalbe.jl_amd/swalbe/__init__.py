@@ -14,7 +14,7 @@ from ._lib import DomainError, SwalbeError, device_count, lib
 from .core import (BGKandStream, DeviceArray, HipState, Sys, SysConst, Taumucs, cospi, equilibrium,
                    fast_32, fast_93, filmpressure, fluid_dry, force_sum, grad_f, grad_f_simple,
                    h_grad_p, h_grad_p_raw, inclination, laplacian_f, measure, moments, power_broad,
-                   slippage, surfacearea, t0, thermal, time_loop, wetted)
+                   slippage, store_mode, surfacearea, t0, thermal, time_loop, wetted)
 from .initialvalues import (Ellipsoid, VPolygon, boxpattern, ellipsepattern, normals, randinterface,
                             random_film, rivulet, sinewave, sinewave1D, singledroplet, torus,
                             trianglepattern, two_droplets_2d, uniforms)

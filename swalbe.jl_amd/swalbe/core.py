@@ -581,13 +581,27 @@ def t0(hᵦ: float = 0.07, γ: float = 0.01, μ: float = 1 / 6, θ: float = 1 / 
     return (3 * μ) / (γ * (qsq * qsq)), qsq
 
 
+def store_mode(Lx: int, Ly: int, fp64: bool, final_outputs_only: bool = False) -> int:
+    """The store mode (swalbe_step_opts.nontemporal) measured fastest for a
+    film on MI355X; every mode gives the same bits.  FP64 films: 2 (f and p
+    stream past the caches).  FP32 films with more than 32 MB of inputs (h, ux,
+    uy beyond the eight 4 MB L2s): 1 (every output streams) when every step
+    writes its outputs, 0 (all cached) when only the last step does; smaller
+    FP32 films the other way round."""
+    if fp64:
+        return 2
+    big = Lx * Ly * 12 > (32 << 20)
+    return (0 if final_outputs_only else 1) if big else (1 if final_outputs_only else 0)
+
+
 def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, thermal: bool = False,
               seed: int = 0, materialize: bool = True, tile_y: int = 0, mass: Optional[list] = None,
               snapshots: Optional[dict] = None, snapshot_every: int = 0,
               maxheight: Optional[list] = None, area_lv: Optional[list] = None,
               red_energy: Optional[list] = None, fluid: Optional[np.ndarray] = None,
               measure_thresh: float = 0.055, surface_theta_field: bool = False,
-              final_outputs_only: bool = False, variant: int = 0, **kw1d):
+              final_outputs_only: bool = False, variant: int = 0,
+              nontemporal: Optional[int] = None, **kw1d):
     """time_loop(sys, state), time_loop(sys, state, θ), time_loop(sys, state, Δh)
     and time_loop(sys, state, f, measure) (src/simulate.jl:6-96).
 
@@ -608,7 +622,9 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     distributions and the pressure on the last step only: the same results
     with 28 instead of 64 bytes per node and step in FP32.  A 1D state runs
     the 1D loop (its own keywords:
-    variant, γ, grad_gamma_sign, snapshots + dumping)."""
+    variant, γ, grad_gamma_sign, snapshots + dumping).  nontemporal
+    sets the stores' cache policy (0, 1, 2: the same bits); by default
+    store_mode() picks the one measured fastest for the film."""
     if _is1d(state):
         return _one_d().time_loop(sys, state, *args, verbose=verbose, thermal=thermal, seed=seed,
                                   materialize=materialize, mass=mass, snapshots=snapshots,
@@ -621,6 +637,8 @@ def time_loop(sys: SysConst, state: HipState, *args, verbose: bool = False, ther
     o = _lib.StepOpts()
     o.thermal, o.seed, o.t0, o.materialize, o.tile_y = int(thermal), int(seed), int(state.t), int(materialize), tile_y
     o.variant = int(variant)  # FP32 kernel: 0 default, 1 / 2 LDS tiles, 3 strips
+    o.nontemporal = int(nontemporal) if nontemporal is not None else store_mode(
+        sys.Lx, sys.Ly, state.T is np.float64, final_outputs_only)
     keep = []
     dh = None
     f = measure = None

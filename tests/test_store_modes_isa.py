@@ -83,3 +83,18 @@ def test_1d_step_streams_f_and_p(stores):
     # per output node: 3 f and p streaming, h and u cached
     for n, nt in pick(stores, "k1_step"):
         assert n % 6 == 0 and 3 * nt == 2 * n, (n, nt)
+
+
+def test_store_mode_rule():
+    """The default store mode of time_loop and bench.py: the mode measured
+    fastest for the film and the pass."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "swalbe.jl_amd"))
+    import swalbe as S
+    assert S.store_mode(4096, 4096, False, False) == 1  # large FP32, every output written
+    assert S.store_mode(4096, 4096, False, True) == 0   # large FP32, final outputs only
+    assert S.store_mode(1024, 1024, False, False) == 0  # 12.6 MB of inputs: stays in L2
+    assert S.store_mode(1024, 1024, False, True) == 1
+    assert S.store_mode(4096, 4096, True) == S.store_mode(64, 48, True, True) == 2
+    # the boundary: 32 MB of FP32 inputs (h, ux, uy)
+    assert S.store_mode(1024, 2731, False) == 1 and S.store_mode(1024, 2730, False) == 0
