@@ -436,7 +436,7 @@ def solve_alone(r: Request, mode: str = "fused", T=np.float32,
 
     from . import _lib
     from .core import (BGKandStream, Sys, SysConst, equilibrium, filmpressure, force_sum,
-                       h_grad_p, moments, slippage)
+                       h_grad_p, moments, slippage, store_mode)
     sysc = SysConst(r.Lx, r.Ly, r.param)
     st = Sys(sysc, "GPU", T=T, kind="thermal" if r.init.thermal else "simple")
     st.set_reference_height(r.init.href, r.param.g)
@@ -450,6 +450,8 @@ def solve_alone(r: Request, mode: str = "fused", T=np.float32,
         o.thermal, o.seed = int(r.init.thermal), int(r.init.seed)
         o.variant, o.tile_y = BATCH_VARIANT, BATCH_TILE_Y
         o.final_outputs_only = int(bool(final_outputs_only))
+        # the store mode measured fastest for the film alone (the same bits in every mode)
+        o.nontemporal = store_mode(r.Lx, r.Ly, T is np.float64, bool(final_outputs_only))
         o.body = int(r.init.body)
         o.body_ax, o.body_ay, o.body_fac = r.init.body_ax, r.init.body_ay, r.init.body_fac
         prm = r.param.cparams()
