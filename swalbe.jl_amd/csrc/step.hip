@@ -106,8 +106,19 @@ __device__ __forceinline__ T force_term(int k, T Fx, T Fy) {
 // forms into one plain store.
 template <bool NT, typename T>
 __device__ __forceinline__ void store_t(T* p, T v) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
+  using G = __attribute__((address_space(1))) T;
+  if constexpr (NT) __builtin_nontemporal_store(v, (G*)p);
+  else *(G*)p = v;
+}
+
+// p, a pointer uniform over the wave, as the compiler's scalar registers hold
+// it: an address formed from it plus a 32-bit lane offset is a store's scalar
+// base and offset, with no 64-bit vector add per store.
+__device__ __forceinline__ __attribute__((address_space(1))) char* scalar_base(const void* p) {
+  const uint64_t a = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return (__attribute__((address_space(1))) char*)(((uint64_t)hi << 32) | lo);
 }
 
 // NTS: 0 every output cached, 1 every output streaming, 2 the distributions
@@ -304,7 +315,7 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   __builtin_amdgcn_s_setprio(1);  // the stores ahead of other waves' pressure and forces
   // ---- phase D: collide the source populations, pull-stream, moments
   const int tx = tid & (TX - 1);
-  const int ty = tid >> 6;
+  const int ty = __builtin_amdgcn_readfirstlane(tid >> 6);  // the wave's rows: uniform
   const T omeg = (T)sp.omeg, itau = (T)sp.itau;
   const T c56g = (T)((5.0 / 6.0) * sp.g), g0 = (T)(1.5 * sp.g);
   double lmax = -__builtin_inf(), lmin = __builtin_inf();
@@ -314,7 +325,16 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   for (int row = ty; row < TY; row += NT / TX) {
     const int gyo = y0 + row;
     if (gxo >= Lx || gyo >= Ly) continue;
-    const int64_t xo = gxo + (int64_t)Lx * gyo;
+    // stores: the output row's first node is a scalar base (uniform over the
+    // wave), the lane's byte offset a 32-bit value made opaque beside each
+    // group of stores (hoisted, it becomes a 64-bit vector operand)
+    const int64_t rowo = (int64_t)Lx * gyo + x0;
+    auto lane_off = [&]() {
+      uint32_t lo = (uint32_t)sizeof(T) * (uint32_t)tx;
+      asm volatile("" : "+v"(lo));
+      return lo;
+    };
+    auto at = [&](T* plane, uint32_t lo) { return (T*)(scalar_base(plane + rowo) + lo); };
     T fo[kQ];
 #pragma unroll
     for (int k = 0; k < kQ; ++k) {
@@ -336,16 +356,23 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
       }
       if (k > 0) f = f + force_term<T>(k, sh_fx[su], sh_fy[su]);
       fo[k] = f;
-      if (io.f_out) store_t<NTS != 0>(&io.f_out[k * FS + xo], f);
+    }
+    if (io.f_out) {
+      const uint32_t lo = lane_off();
+#pragma unroll
+      for (int k = 0; k < kQ; ++k) store_t<NTS != 0>(at(io.f_out + k * FS, lo), fo[k]);
     }
     T s = fo[0];
 #pragma unroll
     for (int k = 1; k < kQ; ++k) s = s + fo[k];
     const T hnew = DEV ? href + s : s;
-    store_t<NTS == 1>(&io.h_out[xo], s);
-    store_t<NTS == 1>(&io.ux_out[xo], (((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7]) + fo[8]) / hnew);
-    store_t<NTS == 1>(&io.uy_out[xo], (((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7]) - fo[8]) / hnew);
-    if (io.p_out) store_t<NTS != 0>(&io.p_out[xo], sh_p[(row + 2) * PC + (tx + 2)]);
+    {
+      const uint32_t lo = lane_off();
+      store_t<NTS == 1>(at(io.h_out, lo), s);
+      store_t<NTS == 1>(at(io.ux_out, lo), (((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7]) + fo[8]) / hnew);
+      store_t<NTS == 1>(at(io.uy_out, lo), (((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7]) - fo[8]) / hnew);
+    }
+    if (io.p_out) store_t<NTS != 0>(at(io.p_out, lane_off()), sh_p[(row + 2) * PC + (tx + 2)]);
     if (ser.dh_keys || ser.wet) {
       const T hc = sh_h[(row + 3) * HC + (tx + 3)];
       const double hv = DEV ? (double)href + (double)hc : (double)hc;
