@@ -101,6 +101,12 @@ __device__ __forceinline__ T force_term(int k, T Fx, T Fy) {
   }
 }
 
+// The force on one node.
+template <typename T>
+struct Fpair {
+  T x, y;
+};
+
 // A store that streams past the caches (non-temporal) when NT.  The choice is
 // a template argument: behind a run-time branch the compiler merges the two
 // forms into one plain store.
@@ -122,8 +128,11 @@ __device__ __forceinline__ __attribute__((address_space(1))) char* scalar_base(c
 }
 
 // NTS: 0 every output cached, 1 every output streaming, 2 the distributions
-// and p streaming while h, ux, uy (the next step's input) stay cached.
-template <typename T, int TY, bool READ_FT, bool THERMAL, int NTS>
+// and p streaming while h, ux, uy (the next step's input) stay cached.  AL:
+// the (Fx, Fy) planes overwrite p once every wave has read it (a barrier
+// between; the forces wait in registers), so p and F share one LDS buffer:
+// 29 KB instead of 35.5 KB for a 64 x 8 tile, 5 workgroups per CU instead of 4.
+template <typename T, int TY, bool READ_FT, bool THERMAL, int NTS, bool AL>
 __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSeries ser,
                                              uint64_t step, int Lx, int Ly, int ntx, int ntiles,
                                              int small, int64_t FS, Enc enc) {
@@ -131,10 +140,14 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   constexpr int HC = TX + 6, HR = TY + 6;  // h tile, 3-halo
   constexpr int PC = TX + 4, PR = TY + 4;  // p tile, 2-halo
   constexpr int UC = TX + 2, UR = TY + 2;  // u and F tiles, 1-halo
+  constexpr int UN = UR * UC, PN = PR * PC;
+  constexpr int PFN = AL ? (PN > 2 * UN ? PN : 2 * UN) : PN + 2 * UN;
   __shared__ T sh_h[HR * HC];
-  __shared__ T sh_p[PR * PC];
+  __shared__ T sh_pf[PFN];  // p, and (Fx, Fy): after p when !AL, over it when AL
   __shared__ T sh_ux[UR * UC], sh_uy[UR * UC];
-  __shared__ T sh_fx[UR * UC], sh_fy[UR * UC];
+  T* const sh_p = sh_pf;
+  T* const sh_fx = AL ? sh_pf : sh_pf + PN;
+  T* const sh_fy = sh_fx + UN;
   __shared__ unsigned long long red_k[2 * (NT / 64)];
   __shared__ unsigned int red_c[NT / 64];
 
@@ -245,7 +258,20 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   }
   __syncthreads();
 
-  // ---- phase C: forces on the 1-halo (src/forcing.jl:42-46, 168-187, 297-311, 363-368)
+  // ---- phase C: forces on the 1-halo (src/forcing.jl:42-46, 168-187, 297-311, 363-368).
+  // With AL, p at this thread's phase D nodes waits in registers and is
+  // stored after the block's last barrier.
+  constexpr int NRW = TY / (NT / TX);  // phase D rows per thread
+  T pown[NRW];
+  auto keep_p = [&]() {
+#pragma unroll
+    for (int j = 0; j < NRW; ++j)
+      pown[j] = sh_p[(tid / TX + j * (NT / TX) + 2) * PC + (tid % TX + 2)];
+  };
+  auto put = [&](int idx, const Fpair<T>& F) {
+    sh_fx[idx] = F.x;
+    sh_fy[idx] = F.y;
+  };
   {
     const T c6mu = (T)sp.c6mu, c6d = (T)sp.c6d, c3dd = (T)sp.c3dd;
     auto fnode = [&](int r, int c, double z0, double z1) {
@@ -273,14 +299,17 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
         Fx = Fx + (hh * (T)sp.bax) * (T)sp.bfac;
         Fy = Fy + (hh * (T)sp.bay) * (T)sp.bfac;
       }
-      sh_fx[idx] = Fx;
-      sh_fy[idx] = Fy;
+      if constexpr (!AL) {
+        sh_fx[idx] = Fx;
+        sh_fy[idx] = Fy;
+      }
+      return Fpair<T>{Fx, Fy};
     };
     if (!small && (Ly & 1) == 0) {
       // one Philox call per vertical node pair (x, 2m), (x, 2m+1): pair row
       // q covers halo rows 2q-1 (even y) and 2q (odd y); y0 is even
-      constexpr int NPR = TY / 2 + 2;
-      for (int t = tid; t < NPR * UC; t += NT) {
+      constexpr int NPR = TY / 2 + 2, NIT = (NPR * UC + NT - 1) / NT;
+      auto pair = [&](int t, Fpair<T>* Fp) {
         const int q = t / UC, c = t - q * UC;
         const int re = 2 * q - 1, ro = 2 * q;
         double z[4] = {0.0, 0.0, 0.0, 0.0};
@@ -293,11 +322,33 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
           box_muller(u32(w[0]), u32(w[1]), &z[0], &z[1]);
           box_muller(u32(w[2]), u32(w[3]), &z[2], &z[3]);
         }
-        if (re >= 0) fnode(re, c, z[0], z[1]);
-        if (ro < UR) fnode(ro, c, z[2], z[3]);
+        if (re >= 0) Fp[0] = fnode(re, c, z[0], z[1]);
+        if (ro < UR) Fp[1] = fnode(ro, c, z[2], z[3]);
+      };
+      if constexpr (AL) {
+        Fpair<T> Fq[NIT][2];
+#pragma unroll
+        for (int i = 0; i < NIT; ++i)
+          if (tid + i * NT < NPR * UC) pair(tid + i * NT, Fq[i]);
+        keep_p();
+        __syncthreads();  // every wave has read p
+#pragma unroll
+        for (int i = 0; i < NIT; ++i) {
+          const int t = tid + i * NT;
+          if (t < NPR * UC) {
+            const int q = t / UC, c = t - q * UC, re = 2 * q - 1, ro = 2 * q;
+            if (re >= 0) put(re * UC + c, Fq[i][0]);
+            if (ro < UR) put(ro * UC + c, Fq[i][1]);
+          }
+        }
+      } else {
+        for (int t = tid; t < NPR * UC; t += NT) {
+          Fpair<T> Fp[2];  // fnode has stored them
+          pair(t, Fp);
+        }
       }
     } else {
-      for (int idx = tid; idx < UR * UC; idx += NT) {
+      auto node = [&](int idx) {
         const int r = idx / UC, c = idx - r * UC;
         double z0 = 0.0, z1 = 0.0;
         if constexpr (THERMAL) {
@@ -306,7 +357,21 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
           gyy = small ? wrapm(gyy, Ly) : wrap1(gyy, Ly);
           normal_pair(sp.seed, step, noise_index(gxx, gyy, Lx), kTagThermal, &z0, &z1);
         }
-        fnode(r, c, z0, z1);
+        return fnode(r, c, z0, z1);
+      };
+      if constexpr (AL) {
+        constexpr int NIU = (UN + NT - 1) / NT;
+        Fpair<T> Fn[NIU];
+#pragma unroll
+        for (int i = 0; i < NIU; ++i)
+          if (tid + i * NT < UN) Fn[i] = node(tid + i * NT);
+        keep_p();
+        __syncthreads();  // every wave has read p
+#pragma unroll
+        for (int i = 0; i < NIU; ++i)
+          if (tid + i * NT < UN) put(tid + i * NT, Fn[i]);
+      } else {
+        for (int idx = tid; idx < UN; idx += NT) node(idx);
       }
     }
   }
@@ -321,6 +386,19 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
   double lmax = -__builtin_inf(), lmin = __builtin_inf();
   unsigned int lcnt = 0;
   const int gxo = x0 + tx;
+  if constexpr (AL) {
+    if (io.p_out) {
+#pragma unroll
+      for (int j = 0; j < NRW; ++j) {
+        const int gyo = y0 + ty + j * (NT / TX);
+        if (gxo < Lx && gyo < Ly) {
+          uint32_t lo = (uint32_t)sizeof(T) * (uint32_t)tx;  // a 32-bit store offset
+          asm volatile("" : "+v"(lo));
+          store_t<NTS != 0>((T*)(scalar_base(io.p_out + ((int64_t)Lx * gyo + x0)) + lo), pown[j]);
+        }
+      }
+    }
+  }
 #pragma unroll 1
   for (int row = ty; row < TY; row += NT / TX) {
     const int gyo = y0 + row;
@@ -372,7 +450,9 @@ __global__ __launch_bounds__(NT) void k_step(StepIO<T> io, StepParams sp, StepSe
       store_t<NTS == 1>(at(io.ux_out, lo), (((((fo[1] - fo[3]) + fo[5]) - fo[6]) - fo[7]) + fo[8]) / hnew);
       store_t<NTS == 1>(at(io.uy_out, lo), (((((fo[2] - fo[4]) + fo[5]) + fo[6]) - fo[7]) - fo[8]) / hnew);
     }
-    if (io.p_out) store_t<NTS != 0>(at(io.p_out, lane_off()), sh_p[(row + 2) * PC + (tx + 2)]);
+    if constexpr (!AL) {
+      if (io.p_out) store_t<NTS != 0>(at(io.p_out, lane_off()), sh_p[(row + 2) * PC + (tx + 2)]);
+    }
     if (ser.dh_keys || ser.wet) {
       const T hc = sh_h[(row + 3) * HC + (tx + 3)];
       const double hv = DEV ? (double)href + (double)hc : (double)hc;
@@ -420,9 +500,19 @@ void launch_ty(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64
   const int ntiles = ntx * nty;
   const int small = (Lx < TX + 6 || Ly < TY + 6) ? 1 : 0;
   dim3 grid(ntiles), block(NT);
-#define SW_K(RF, TH, M)                                                                      \
-  hipLaunchKernelGGL((k_step<T, TY, RF, TH, M>), grid, block, 0, s, io, sp, ser, step, Lx, Ly, \
-                     ntx, ntiles, small, FS, enc)
+  // The shared p / F buffer lifts a 64 x 8 tile from 4 to 5 workgroups per CU
+  // for one more barrier per tile: taken where the film has more tiles than
+  // 4 per CU hold at once.
+  const bool al = (int64_t)ntiles > 4 * (int64_t)device_cus();
+#define SW_K(RF, TH, M)                                                                        \
+  do {                                                                                         \
+    if (al)                                                                                    \
+      hipLaunchKernelGGL((k_step<T, TY, RF, TH, M, true>), grid, block, 0, s, io, sp, ser, step, \
+                         Lx, Ly, ntx, ntiles, small, FS, enc);                                 \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_step<T, TY, RF, TH, M, false>), grid, block, 0, s, io, sp, ser,     \
+                         step, Lx, Ly, ntx, ntiles, small, FS, enc);                           \
+  } while (0)
 #define SW_M(RF, TH)          \
   do {                        \
     if (nts == 2) SW_K(RF, TH, 2); \

@@ -925,22 +925,6 @@ __global__ __launch_bounds__(NTH) void k_step_f32_unwritten(TileIO io, F32K K, T
                                                         red_c);
 }
 
-// Compute units of the current device, asked once per device (256 if the
-// runtime cannot say).
-int device_cus() {
-  static int cus[64];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int c = 0;
-    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        c <= 0)
-      c = 256;
-    cus[dev] = c;
-  }
-  return cus[dev];
-}
-
 template <int TXW, int NTH, int TY, bool SRC>
 void launch_single(const StepIO<float>& io, const StepParams& sp, StepSeries ser, uint64_t step,
                    int Lx, int Ly, int64_t fstride, Enc enc, int nts, hipStream_t s) {

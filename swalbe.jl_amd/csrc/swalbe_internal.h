@@ -34,6 +34,22 @@ __device__ __forceinline__ int tile_of_block(int ntx, int ntiles, uint64_t step)
   return row * ntx + x * cpr + k;
 }
 
+// Compute units of the current device, asked once per device (256 if the
+// runtime cannot say).
+inline int device_cus() {
+  static int cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        c <= 0)
+      c = 256;
+    cus[dev] = c;
+  }
+  return cus[dev];
+}
+
 // The FP64 step's order for a film whose inputs (h, ux, uy) fit the eight
 // L2s (4 MB each, so in_bytes <= 32 MB) and whose tile count is a multiple
 // of 8: XCD x takes the x-th eighth of the tile list, a band of rows, and
