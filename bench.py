@@ -734,7 +734,7 @@ def main():
                                + ("rupture with thermal noise, kbt=1e-6, " if thermal else "without noise, ")
                                + f"{Lx}x{Ly} periodic, theta=1/9, n=9, m=3, gamma=0.01, tau=1",
                    "Lx": Lx, "Ly": Ly, "parallelism": "single GPU",
-                   "tile_y": a.tile_y or (16 if dt == "f32" else 8),
+                   "tile_y": a.tile_y or (16 if dt == "f32" or Lx * Ly * 24 > (32 << 20) else 8),
                    "variant": a.variant, "nontemporal": nt_full,
                    "plane_pad": int(os.environ.get("SWALBE_PLANE_PAD", "0")), "finite": ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -154,7 +154,8 @@ typedef struct swalbe_step_opts {
     double *dh_series;     /* host [nsteps]: max(h) - min(h) before each step (:56), or NULL */
     int64_t *wet_series;   /* host [nsteps]: count(h > wet_thresh) (measures.jl:13-17), or NULL */
     double *mass_series;   /* host [nsteps/tdump + 1]: sum(h) when t % tdump == 0 (:8-14), or NULL */
-    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16 for FP32, 8 for FP64);
+    int32_t tile_y;        /* rows per workgroup tile: 0 = default (16 for FP32; FP64 16 where
+                              h, ux, uy exceed 32 MB, else 8);
                               8 or 16 (the same results either way); variant 3: rows per
                               strip segment, 0 = default (64), 4..4096 */
     int32_t variant;       /* FP32 kernel: 0 = default (2), 1 = LDS tiles, collide at the

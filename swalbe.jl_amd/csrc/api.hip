@@ -933,11 +933,14 @@ int run_loop(swalbe_state* st, const swalbe_params* p, int64_t nsteps, const swa
     return SWALBE_EINVAL;
   }
   const bool strips = o->variant == 3;
-  // default tile height: 16 rows for FP32; 8 for FP64, whose 16-row tile
-  // (61 KB of LDS) leaves two workgroups per CU (measured 27.6 against 26.8
-  // GLUPS, and 34.2 against 30.3 with the final outputs only)
-  const int tile_y =
-      o->tile_y > 0 ? o->tile_y : (strips ? kStripSeg : (F.dtype == SWALBE_F64 ? 8 : 16));
+  // default tile height: 16 rows for FP32.  FP64: 16 rows where the film's
+  // inputs (h, ux, uy) exceed the eight 4 MB L2s (the tile then shares one LDS
+  // buffer between p and the forces, 3 workgroups per CU: 32.2 against 30.8
+  // GLUPS at 4096²), else 8 (38.4 against 33.8 on a 1024² film)
+  const bool f64_big = (int64_t)F.Lx * F.Ly * 24 > ((int64_t)32 << 20);
+  const int tile_y = o->tile_y > 0 ? o->tile_y
+                                   : (strips ? kStripSeg
+                                             : (F.dtype == SWALBE_F64 ? (f64_big ? 16 : 8) : 16));
   if (strips ? (tile_y < 4 || tile_y > 4096) : (tile_y != 8 && tile_y != 16)) {
     set_error("swalbe_time_loop: tile_y must be 8 or 16 (4..4096 rows per segment for variant 3)");
     return SWALBE_EINVAL;

@@ -501,9 +501,10 @@ void launch_ty(const StepIO<T>& io, const StepParams& sp, StepSeries ser, uint64
   const int small = (Lx < TX + 6 || Ly < TY + 6) ? 1 : 0;
   dim3 grid(ntiles), block(NT);
   // The shared p / F buffer lifts a 64 x 8 tile from 4 to 5 workgroups per CU
-  // for one more barrier per tile: taken where the film has more tiles than
-  // 4 per CU hold at once.
-  const bool al = (int64_t)ntiles > 4 * (int64_t)device_cus();
+  // (64 x 16: from 2 to 3) for one more barrier per tile: taken where the film
+  // has more tiles than the CUs hold at once without it.
+  constexpr int kResident = TY == 8 ? 4 : 2;
+  const bool al = (int64_t)ntiles > kResident * (int64_t)device_cus();
 #define SW_K(RF, TH, M)                                                                        \
   do {                                                                                         \
     if (al)                                                                                    \

@@ -231,6 +231,31 @@ def test_wide_films_fp64_bitwise(S, shape):
     st.close()
 
 
+@pytest.mark.parametrize("tile_y", [8, 16])
+def test_fp64_shared_buffer_bitwise(S, tile_y):
+    """A film with more FP64 tiles than the CUs hold at once without it
+    (2048 x 1024: 4096 tiles of 64 x 8, 2048 of 64 x 16) shares one LDS buffer
+    between p and the forces, which wait in registers across one more barrier;
+    p is stored after the last one.  FP64 bit-exact against the oracle over 6
+    steps with thermal noise, both tile heights (16 is this film's default)."""
+    Lx, Ly, nsteps = 2048, 1024, 6
+    print(f"[parity] FP64 shared buffer {Lx}x{Ly} tile_y {tile_y}", flush=True)
+    h0 = O.init_normal(Lx, Ly, 21, 1.0, 0.01)
+    prm = O.params(gamma=0.01, kbt=1e-5, tdump=10 ** 9)
+    ost = _oracle_state(h0)
+    O.time_loop(ost, prm, nsteps, thermal=True, seed=5)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=10 ** 9, γ=0.01, kbt=1e-5))
+    st = S.Sys(sysc, "GPU", kind="thermal")
+    st.height = h0
+    S.equilibrium(st, sysc)
+    S.time_loop(sysc, st, thermal=True, seed=5, tile_y=tile_y)
+    for name, ref in (("height", ost.height), ("velx", ost.velx), ("vely", ost.vely),
+                      ("pressure", ost.pressure), ("fout", ost.fout)):
+        got = st.get(name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {relL2(got, ref):.3e}"
+    st.close()
+
+
 def test_wide_film_fp32_tolerance(S):
     """The FP32 step on a film 64 tiles wide (the XCD column-group order):
     rel-L2(h) against the FP64 oracle <= 1e-5 after 41 steps with noise."""
