@@ -121,7 +121,9 @@ def test_large_film_alone_matches_batch(S):
     1024: 2048 tiles) steps alone with p and the (Fx, Fy) pairs sharing one
     LDS buffer; the batch kernel keeps the two apart.  Both give the same
     bits: τ = 1 with noise and τ = 0.8, alone with every output written each
-    step and with the final outputs only."""
+    step and with the final outputs only.  1500 x 1031 (1560 tiles) adds
+    partial tiles on the right and top edges and an odd Ly (one noise call
+    per node)."""
     from swalbe import serve
     T, fi = S.Taumucs, S.film_init
     specs = [
@@ -130,6 +132,8 @@ def test_large_film_alone_matches_batch(S):
         (2048, 1024, T(Tmax=1, τ=0.8, γ=0.01, θ=1 / 9), fi(S.INIT_CAP, precursor=0.1, href=0.1,
                                                           cap1=(150.0, 1 / 9, 1024.5, 512.5)),
          30),
+        (1500, 1031, T(Tmax=1, kbt=1e-6, γ=0.01), fi(S.INIT_NORMAL, thermal=True, seed=23,
+                                                     base=1.0, amp=1e-3, href=1.0), 25),
     ]
     b = S.Batch(1 << 30, 8)
     slots = [b.add(prm, Lx, Ly, init, n) for Lx, Ly, prm, init, n in specs]

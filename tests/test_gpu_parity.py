@@ -231,14 +231,19 @@ def test_wide_films_fp64_bitwise(S, shape):
     st.close()
 
 
+@pytest.mark.parametrize("shape", [(2048, 1024), (1000, 1031)], ids=lambda s: f"{s[0]}x{s[1]}")
 @pytest.mark.parametrize("tile_y", [8, 16])
-def test_fp64_shared_buffer_bitwise(S, tile_y):
+def test_fp64_shared_buffer_bitwise(S, tile_y, shape):
     """A film with more FP64 tiles than the CUs hold at once without it
     (2048 x 1024: 4096 tiles of 64 x 8, 2048 of 64 x 16) shares one LDS buffer
     between p and the forces, which wait in registers across one more barrier;
     p is stored after the last one.  FP64 bit-exact against the oracle over 6
-    steps with thermal noise, both tile heights (16 is this film's default)."""
-    Lx, Ly, nsteps = 2048, 1024, 6
+    steps with thermal noise, both tile heights (16 is this film's default).
+    1000 x 1031 (2064 and 1040 tiles) has partial tiles on the right and top
+    edges, an odd Ly (one noise call per node) and 24.7 MB of inputs, so its
+    tiles take the XCD-banded order."""
+    Lx, Ly = shape
+    nsteps = 6
     print(f"[parity] FP64 shared buffer {Lx}x{Ly} tile_y {tile_y}", flush=True)
     h0 = O.init_normal(Lx, Ly, 21, 1.0, 0.01)
     prm = O.params(gamma=0.01, kbt=1e-5, tdump=10 ** 9)
