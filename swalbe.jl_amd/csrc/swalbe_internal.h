@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 
 #include "swalbe_hip.h"
@@ -35,19 +36,20 @@ __device__ __forceinline__ int tile_of_block(int ntx, int ntiles, uint64_t step)
 }
 
 // Compute units of the current device, asked once per device (256 if the
-// runtime cannot say).
+// runtime cannot say).  Threads that race on the first call store the same
+// value; the atomics keep that race defined.
 inline int device_cus() {
-  static int cus[64];
+  static std::atomic<int> cus[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cus[dev] == 0) {
-    int c = 0;
+  int c = cus[dev].load(std::memory_order_relaxed);
+  if (c == 0) {
     if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         c <= 0)
       c = 256;
-    cus[dev] = c;
+    cus[dev].store(c, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return c;
 }
 
 // The FP64 step's order for a film whose inputs (h, ux, uy) fit the eight
