@@ -281,6 +281,34 @@ def test_resident_nodes_per_thread_bitwise_vs_oracle(S, L, name, threads):
         assert np.array_equal(np.array(dh), out["dh"])
 
 
+@pytest.mark.parametrize("threads", [256, 512])
+@pytest.mark.parametrize("mask", ["fractional", "negative_zero"])
+def test_resident_masks_of_any_value_bitwise_vs_oracle(S, mask, threads):
+    """Wall masks other than the 0 and 1 that obslist! makes (a 0.5 and a
+    0.25, or -0 whose products keep their sign) in the resident kernel, with
+    the masks of each thread's first two nodes held in registers and the
+    rest read from global memory: the oracle's bits."""
+    L, n = 1500, 120
+    print(f"1D resident masks {mask} L={L} threads {threads}", flush=True)
+    sysc, prm, st, ost, okw, hkw, args = _setup(S, L, n, CONFIGS["walls"], variant=2)
+    b1, b2 = sysc.border[0], sysc.border[1]
+    if mask == "fractional":
+        b1[L // 4] = 0.5
+        b2[L // 4 + 1] = 0.25
+    else:
+        b1[np.flatnonzero(b1 == 0)[:7]] = -0.0
+    st.set("border1", b1)
+    st.set("border2", b2)
+    okw["borders"] = (b1.copy(), b2.copy())
+    hkw["threads"] = threads
+    out = O.time_loop_1d(ost, prm, n, **okw)
+    dh = _run_hip(S, sysc, st, hkw, args, None)
+    for f_h, f_o in (("height", "height"), ("vel", "vel"), ("fout", "fout"),
+                     ("pressure", "pressure")):
+        assert np.array_equal(st.get(f_h), getattr(ost, f_o)), (mask, f_h)
+    assert np.array_equal(np.array(dh), out["dh"])
+
+
 @pytest.mark.parametrize("L", [3, 255, 257, 1000])
 def test_fp64_lengths_and_both_loops_agree(S, L):
     print(f"1D lengths L={L}", flush=True)
