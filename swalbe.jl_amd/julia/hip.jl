@@ -65,12 +65,12 @@ HipStepOpts(; use_theta_field = 0, thermal = 0, body = 0, materialize = 1, seed 
              surf_theta_field, final_outputs_only)
 
 # The store mode (nontemporal; every mode gives the same bits) measured fastest
-# for a film on MI355X, as swalbe.store_mode of the Python mirror: FP64 films 2
-# when every step writes its outputs, 0 when only the last step does; FP32
-# films with more than 32 MB of inputs 1 and 0; smaller FP32 films 0 and 1.
+# for a film on MI355X, as swalbe.store_mode of the Python mirror: FP64 films 2;
+# FP32 films with more than 32 MB of inputs 1 when every step writes its
+# outputs, 0 when only the last step does; smaller FP32 films the other way.
 function store_mode(Lx, Ly, ::Type{T}, final_outputs_only) where {T}
+    T === Float64 && return Int32(2)
     last = final_outputs_only != 0
-    T === Float64 && return Int32(last ? 0 : 2)
     return Int32(Lx * Ly * 12 > (32 << 20) ? (last ? 0 : 1) : (last ? 1 : 0))
 end
 with_store_mode(o::HipStepOpts, mode) =

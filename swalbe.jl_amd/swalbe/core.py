@@ -584,13 +584,12 @@ def t0(hᵦ: float = 0.07, γ: float = 0.01, μ: float = 1 / 6, θ: float = 1 / 
 def store_mode(Lx: int, Ly: int, fp64: bool, final_outputs_only: bool = False) -> int:
     """The store mode (swalbe_step_opts.nontemporal) measured fastest for a
     film on MI355X; every mode gives the same bits.  FP64 films: 2 (f and p
-    stream past the caches) when every step writes its outputs, 0 (all
-    cached) when only the last step does.  FP32 films with more than 32 MB of
-    inputs (h, ux, uy beyond the eight 4 MB L2s): 1 (every output streams)
-    when every step writes its outputs, 0 when only the last step does;
-    smaller FP32 films the other way round."""
+    stream past the caches).  FP32 films with more than 32 MB of inputs (h, ux,
+    uy beyond the eight 4 MB L2s): 1 (every output streams) when every step
+    writes its outputs, 0 (all cached) when only the last step does; smaller
+    FP32 films the other way round."""
     if fp64:
-        return 0 if final_outputs_only else 2
+        return 2
     big = Lx * Ly * 12 > (32 << 20)
     return (0 if final_outputs_only else 1) if big else (1 if final_outputs_only else 0)
 

@@ -96,7 +96,6 @@ def test_store_mode_rule():
     assert S.store_mode(4096, 4096, False, True) == 0   # large FP32, final outputs only
     assert S.store_mode(1024, 1024, False, False) == 0  # 12.6 MB of inputs: stays in L2
     assert S.store_mode(1024, 1024, False, True) == 1
-    assert S.store_mode(4096, 4096, True) == S.store_mode(64, 48, True) == 2  # FP64
-    assert S.store_mode(4096, 4096, True, True) == S.store_mode(64, 48, True, True) == 0
+    assert S.store_mode(4096, 4096, True) == S.store_mode(64, 48, True, True) == 2
     # the boundary: 32 MB of FP32 inputs (h, ux, uy)
     assert S.store_mode(1024, 2731, False) == 1 and S.store_mode(1024, 2730, False) == 0
