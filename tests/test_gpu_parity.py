@@ -261,6 +261,30 @@ def test_fp64_shared_buffer_bitwise(S, tile_y, shape):
     st.close()
 
 
+def test_full_size_4096_fp64_bitwise(S):
+    """The 4096 x 4096 rupture film at full size (16-row tiles with the
+    shared LDS buffer, 16384 tiles in the XCD column-group order, forwards
+    and backwards): FP64 bit-exact against the oracle over 3 steps with
+    thermal noise."""
+    Lx = Ly = 4096
+    nsteps = 3
+    print(f"[parity] full size {Lx}x{Ly} FP64, {nsteps} steps", flush=True)
+    h0 = O.init_normal(Lx, Ly, 1234, 1.0, 0.001)
+    prm = O.params(gamma=0.01, kbt=1e-6, tdump=10 ** 9)
+    ost = _oracle_state(h0)
+    O.time_loop(ost, prm, nsteps, thermal=True, seed=7)
+    sysc = S.SysConst(Lx, Ly, S.Taumucs(Tmax=nsteps, tdump=10 ** 9, γ=0.01, kbt=1e-6))
+    st = S.Sys(sysc, "GPU", kind="thermal")
+    st.height = h0
+    S.equilibrium(st, sysc)
+    S.time_loop(sysc, st, thermal=True, seed=7)
+    for name, ref in (("height", ost.height), ("velx", ost.velx), ("vely", ost.vely),
+                      ("pressure", ost.pressure), ("fout", ost.fout)):
+        got = st.get(name)
+        assert np.array_equal(got, ref), f"{name}: rel-L2 {relL2(got, ref):.3e}"
+    st.close()
+
+
 def test_wide_film_fp32_tolerance(S):
     """The FP32 step on a film 64 tiles wide (the XCD column-group order):
     rel-L2(h) against the FP64 oracle <= 1e-5 after 41 steps with noise."""
