@@ -30,7 +30,9 @@ namespace {
 
 constexpr int kNT1 = 256;  // output nodes per workgroup of k1_step and the per-op kernels
 constexpr int kRTMax = 1024;  // most threads per film of k1_resident
-constexpr int64_t kResidentStaticLds = 256;  // k1_resident's static LDS (reduction slots)
+// k1_resident's static LDS: its reduction slots, 4 doubles per wave, at the
+// most threads (512 B); the film's planes take what is left
+constexpr int64_t kResidentStaticLds = 4 * (kRTMax / 64) * (int64_t)sizeof(double);
 
 inline int blocks1d(int64_t n) {
   int64_t b = (n + kNT1 - 1) / kNT1;
@@ -434,6 +436,7 @@ __global__ __launch_bounds__(kRT) __attribute__((amdgpu_waves_per_eu(kRT == 512 
 void k1_resident(const Film1* table, int64_t nsteps) {
   extern __shared__ __align__(16) unsigned char smem1[];
   __shared__ double red[4][kRT / 64];
+  static_assert(kRT <= kRTMax && sizeof(red) <= kResidentStaticLds, "static LDS budget");
   const auto* F1 = glb(table + blockIdx.x);
   const int L = F1->L, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int read_ft = F1->read_ft;

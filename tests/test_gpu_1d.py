@@ -431,6 +431,24 @@ def test_ensemble_two_nodes_per_thread_equals_oracle(S):
         assert np.array_equal(st.pressure, ost.pressure), name
 
 
+@pytest.mark.parametrize("name", ["thermal", "walls", "tau"])
+def test_resident_at_its_maximum_length_bitwise_vs_oracle(S, name):
+    """A film of exactly swalbe1d_resident_max_length nodes (every byte of
+    LDS a workgroup may take; τ ≠ 1 holds f_t too, so its limit is lower)
+    runs resident and gives the oracle's bits."""
+    cfg = CONFIGS[name]
+    L = S.resident_max_length(np.float64, tau_is_one=name != "tau")
+    n = 60
+    print(f"1D resident at the maximum length {name} L={L}", flush=True)
+    sysc, prm, st, ost, okw, hkw, args = _setup(S, L, n, cfg, variant=2)
+    out = O.time_loop_1d(ost, prm, n, **okw)
+    dh = _run_hip(S, sysc, st, hkw, args, None)
+    for f_h, f_o in (("height", "height"), ("vel", "vel"), ("fout", "fout"),
+                     ("pressure", "pressure")):
+        assert np.array_equal(st.get(f_h), getattr(ost, f_o)), (name, L, f_h)
+    assert np.array_equal(np.array(dh), out["dh"])
+
+
 def test_large_film_runs_per_step(S):
     L = 1 << 20
     n = 8
