@@ -449,6 +449,23 @@ def test_resident_at_its_maximum_length_bitwise_vs_oracle(S, name):
     assert np.array_equal(np.array(dh), out["dh"])
 
 
+def test_fp32_resident_at_its_maximum_length(S):
+    """The FP32 limit (half the bytes per node, so twice the FP64 length)
+    runs resident: rel-L2(h) against the FP64 oracle <= 1e-6 after 60 steps."""
+    L = S.resident_max_length(np.float32)
+    assert L > S.resident_max_length(np.float64)
+    n = 60
+    print(f"1D FP32 resident at the maximum length L={L}", flush=True)
+    sysc, prm, st, ost, okw, hkw, args = _setup(S, L, n, CONFIGS["plain"], T=np.float32,
+                                                variant=2)
+    S.time_loop(sysc, st, **hkw)
+    O.time_loop_1d(ost, prm, n, **okw)
+    h = st.get("height", np.float64)
+    rel = np.linalg.norm(h - ost.height) / np.linalg.norm(ost.height)
+    print(f"1D FP32 L={L}: rel-L2 h {rel:.3e}", flush=True)
+    assert np.isfinite(h).all() and rel <= 1e-6
+
+
 def test_large_film_runs_per_step(S):
     L = 1 << 20
     n = 8
