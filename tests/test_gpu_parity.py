@@ -361,6 +361,7 @@ FP32_CASES = [
     ((199, 61), 0, 3, "thermal"),       # odd Lx, odd Ly
     ((40, 20), 0, 3, "thermal"),        # narrower than a strip: modulo wrap
     ((130, 7), 4, 3, "thermal"),        # fewer rows than the halo
+    ((256, 96), 4096, 3, "thermal"),    # the longest segment: one per strip, past Ly
     ((192, 64), 32, 3, "theta"),
     ((192, 64), 64, 3, "tau"),
     ((192, 64), 5, 3, "body"),
