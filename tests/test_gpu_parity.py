@@ -10,7 +10,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(7, 5), (64, 16), (100, 37), (130, 70), (5, 9)]
+SHAPES = [(7, 5), (64, 16), (100, 37), (130, 70), (5, 9), (1, 1), (2, 3)]
 
 
 def relL2(a, b):
@@ -154,7 +154,8 @@ VARIANTS = {
 }
 
 
-@pytest.mark.parametrize("shape", [(70, 45), (64, 64), (7, 5), (130, 19)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("shape", [(70, 45), (64, 64), (7, 5), (130, 19), (1, 1), (2, 3), (3, 1)],
+                         ids=lambda s: f"{s[0]}x{s[1]}")
 @pytest.mark.parametrize("variant", list(VARIANTS), ids=list(VARIANTS))
 def test_fused_loop_variants_fp64(S, shape, variant):
     print(f"[parity] fused {variant} {shape}", flush=True)
@@ -362,6 +363,9 @@ FP32_CASES = [
     ((40, 20), 0, 3, "thermal"),        # narrower than a strip: modulo wrap
     ((130, 7), 4, 3, "thermal"),        # fewer rows than the halo
     ((256, 96), 4096, 3, "thermal"),    # the longest segment: one per strip, past Ly
+    ((2, 3), 16, 2, "thermal"),         # films of 1-3 nodes a side: every neighbour wraps
+    ((1, 4), 8, 1, "thermal"),
+    ((3, 1), 0, 3, "thermal"),
     ((192, 64), 32, 3, "theta"),
     ((192, 64), 64, 3, "tau"),
     ((192, 64), 5, 3, "body"),
