@@ -70,6 +70,9 @@ def film_specs(S):
                                                   precursor=0.1, href=0.1, clamp=True,
                                                   cap1=(20.0, 1 / 12, 32.5, 32.5)), 100),
         (130, 22, T(Tmax=1, γ=0.01), fi(S.INIT_FLAT, base=0.5, href=0.5), 50),
+        # three by two nodes: every neighbour of every node wraps
+        (3, 2, T(Tmax=1, kbt=1e-6, γ=0.01), fi(S.INIT_NORMAL, thermal=True, seed=16, base=1.0,
+                                                amp=1e-3, href=1.0), 40),
     ]
     return specs
 
